@@ -1,0 +1,368 @@
+// Host side of the C ABI declared in include/cos_capon.h.
+//
+// Drop-in entry points take host pointers, stage them through device buffers owned
+// by a process-wide plan (grown on demand, never shrunk; the reference re-allocates
+// on every call because cur_Npx is never updated, RealTimeCapon/Capon.cpp:110-141,
+// :528-547), run on the null stream and copy the outputs back.  Plan entry points
+// take device pointers and a stream and only enqueue work.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cos_capon.h"
+#include "capon_kernels.h"
+#include "components.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+int fail_hip(hipError_t e, const char* where) {
+  g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+  return (int)e;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) n = bytes;
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct cos_capon_plan {
+  DevBuf flags, list, count;  // fallback bookkeeping
+  DevBuf x, z, w, R, P, B;    // drop-in staging
+  hipStream_t last_stream = nullptr;
+  int last_fallback = 0;
+  bool fallback_ran = false;
+  ~cos_capon_plan() {
+    for (DevBuf* b : {&flags, &list, &count, &x, &z, &w, &R, &P, &B}) b->release();
+  }
+};
+
+namespace {
+
+std::mutex g_default_mu;
+cos_capon_plan* g_default = nullptr;
+
+cos_capon_plan* default_plan() {
+  if (!g_default) g_default = new cos_capon_plan();
+  return g_default;
+}
+
+// Argument checks in the reference's order (RealTimeCapon/Capon.cpp:98-107,
+// RealTimeCapon/BuildR.cpp:37-48) plus this library's kernel limits.
+int check_args(float d, int L, int Yavg, int M, int Nx, int Nz, int Nb) {
+  if (Nx < 1 || Nz < 1 || L < 1 || M < 1 || Yavg < 0 || Nb < 0)
+    return fail(hipErrorInvalidValue, "Nx or Nz were < 1, or L or M < 1, or Yavg or Nb < 0");
+  if (Nb > L) return fail(hipErrorInvalidValue, "Nb can not be larger than L");
+  if (L > M) return fail(hipErrorInvalidValue, "L > M");
+  if (2 * Yavg >= Nz) return fail(hipErrorInvalidValue, "2*Yavg >= Nz");
+  if (L > 64) return fail(hipErrorInvalidValue, "L > 64 is not supported");
+  if (!(d >= 0.0f) || !std::isfinite(d)) return fail(hipErrorInvalidValue, "d must be finite and >= 0");
+  if ((long)Nx * Nz * M > (1L << 31) - 1) return fail(hipErrorInvalidValue, "cube too large for one call");
+  return 0;
+}
+
+cosk::CaponLaunch base_launch(const cos_capon_desc& ds) {
+  cosk::CaponLaunch a;
+  std::memset(&a, 0, sizeof(a));
+  a.Nx = ds.Nx;
+  a.Nz = ds.Nz;
+  a.Mstride = ds.Mstride;
+  a.Y = ds.Yavg;
+  a.L = ds.L;
+  a.M = ds.M;
+  a.off = ds.off;
+  a.d = ds.d;
+  a.rowL = ds.row_L;
+  a.rowM = ds.row_M;
+  a.rowOff = ds.row_off;
+  a.rowD = ds.row_d;
+  a.npix = ds.Nx * (ds.Nz - 2 * ds.Yavg);
+  a.cond_thresh = ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6f;
+  a.steer_sin = ds.steer_sin;
+  a.nsteer = ds.nsteer;
+  return a;
+}
+
+// Butler beams (RealTimeCapon/capon_kernel.cu:47-74) in double on the host:
+// B[b][l] = L^-1/2 exp(-j 2 pi beta(b) l / L); beta(b) = b for b <= Nb/2 else
+// b + L - 2(Nb/2) - 1 (the even-Nb +1 at capon_kernel.cu:58 never fires).
+std::vector<float> butler_host(int Nb, int L) {
+  std::vector<float> B(2 * (size_t)Nb * L);
+  const double s = 1.0 / std::sqrt((double)L);
+  for (int b = 0; b < Nb; ++b) {
+    int beta = b > Nb / 2 ? b + L - 2 * (Nb / 2) - 1 : b;
+    for (int l = 0; l < L; ++l) {
+      double ph = -2.0 * M_PI * (double)((long)beta * l % L) / (double)L;
+      B[2 * ((size_t)b * L + l)] = (float)(s * std::cos(ph));
+      B[2 * ((size_t)b * L + l) + 1] = (float)(s * std::sin(ph));
+    }
+  }
+  return B;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cos_capon_last_error(void) { return g_last_error.c_str(); }
+
+int cos_capon_plan_create(cos_capon_plan** plan) {
+  if (!plan) return fail(hipErrorInvalidValue, "plan is NULL");
+  *plan = new cos_capon_plan();
+  return 0;
+}
+
+int cos_capon_plan_destroy(cos_capon_plan* plan) {
+  delete plan;
+  return 0;
+}
+
+int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x, float* P,
+                  cos_cf32* z, cos_cf32* w, int wstride, cos_cf32* R, unsigned char* flags,
+                  void* stream) {
+  g_last_error.clear();
+  if (!plan || !desc || !x) return fail(hipErrorInvalidValue, "plan, desc and x must be non-NULL");
+  const cos_capon_desc& ds = *desc;
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  if (tables && !(ds.row_L && ds.row_M && ds.row_off && ds.row_d))
+    return fail(hipErrorInvalidValue, "row tables must be given together (L, M, off, d)");
+  const int Lmax = tables ? ds.Lmax : ds.L;
+  const int Mmax = tables ? ds.Mmax : ds.M;
+  if (int e = check_args(tables ? 0.f : ds.d, tables ? 1 : ds.L, ds.Yavg, tables ? 1 : ds.M, ds.Nx,
+                         ds.Nz, 0))
+    return e;
+  if (Lmax < 1 || Lmax > 64 || Mmax < Lmax) return fail(hipErrorInvalidValue, "need 1 <= Lmax <= 64, Mmax >= Lmax");
+  if (!tables && ds.off + ds.M > ds.Mstride) return fail(hipErrorInvalidValue, "off + M > Mstride");
+  if (tables && Mmax > ds.Mstride) return fail(hipErrorInvalidValue, "Mmax > Mstride");
+  if (R && tables) return fail(hipErrorInvalidValue, "R output needs a uniform L");
+  if (w && wstride < Lmax) return fail(hipErrorInvalidValue, "wstride < L");
+  if (ds.nsteer < 0 || (ds.nsteer > 0 && !ds.steer_sin)) return fail(hipErrorInvalidValue, "bad steering");
+  hipStream_t s = (hipStream_t)stream;
+
+  cosk::CaponLaunch a = base_launch(ds);
+  a.P = P;
+  a.z = (float2*)z;
+  a.w = (float2*)w;
+  a.wstride = wstride;
+  a.R = (float2*)R;
+  a.Rstride = ds.L * ds.L;
+  const char* why = nullptr;
+  hipError_t e;
+  plan->fallback_ran = false;
+  plan->last_stream = s;
+  if (ds.precision == COS_PREC_FP64) {
+    e = cosk::launch_capon_f64(a, Lmax, Mmax, s, &why);
+    if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 kernel");
+    return 0;
+  }
+  const bool fb = ds.precision == COS_PREC_FP32_FALLBACK;
+  uint8_t* fl = (uint8_t*)flags;
+  if (fb && !fl) {
+    if ((e = plan->flags.ensure(a.npix)) != hipSuccess) return fail_hip(e, "flags buffer");
+    fl = plan->flags.as<uint8_t>();
+  }
+  a.flags = fl;
+  e = cosk::launch_capon_f32(a, Lmax, Mmax, s, &why);
+  if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
+  if (!fb) return 0;
+  if (ds.nsteer > 0) return fail(hipErrorInvalidValue, "FP64 fallback does not support steering");
+  if (Lmax > 32) return fail(hipErrorInvalidValue, "FP64 fallback supports L <= 32");
+  if ((e = plan->list.ensure(sizeof(int) * (size_t)a.npix)) != hipSuccess) return fail_hip(e, "list buffer");
+  if ((e = plan->count.ensure(sizeof(int))) != hipSuccess) return fail_hip(e, "count buffer");
+  e = cosk::compact_flags(fl, a.npix, plan->list.as<int>(), plan->count.as<int>(), s);
+  if (e != hipSuccess) return fail_hip(e, "compaction");
+  cosk::CaponLaunch f = a;
+  f.flags = nullptr;
+  f.plist = plan->list.as<int>();
+  f.pcount = plan->count.as<int>();
+  e = cosk::launch_capon_f64(f, Lmax, Mmax, s, &why);
+  if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 fallback");
+  plan->fallback_ran = true;
+  return 0;
+}
+
+int cos_capon_last_fallback_count(cos_capon_plan* plan, int* count) {
+  if (!plan || !count) return fail(hipErrorInvalidValue, "NULL argument");
+  *count = 0;
+  if (!plan->fallback_ran) return 0;
+  hipError_t e = hipStreamSynchronize(plan->last_stream);
+  if (e != hipSuccess) return fail_hip(e, "synchronize");
+  e = hipMemcpy(count, plan->count.p, sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail_hip(e, "copy count");
+  return 0;
+}
+
+int cos_capon_get_capon_p(cos_cf32* z, cos_cf32* w, cos_cf32* R, float* P, const cos_cf32* x, float d,
+                          int L, int Yavg, int M, int Nx, int Nz, int Nb) {
+  g_last_error.clear();
+  if (int e = check_args(d, L, Yavg, M, Nx, Nz, Nb)) return e;
+  if (!x) return fail(hipErrorInvalidValue, "x is NULL");
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  cos_capon_plan* pl = default_plan();
+  const size_t npx = (size_t)Nx * (Nz - 2 * Yavg);
+  const size_t xb = (size_t)Nx * Nz * M * sizeof(cos_cf32);
+  hipError_t e;
+  const bool needR = R || Nb > 0;
+  if ((e = pl->x.ensure(xb)) != hipSuccess) return fail_hip(e, "device buffer x");
+  if (z && (e = pl->z.ensure(npx * sizeof(cos_cf32))) != hipSuccess) return fail_hip(e, "device buffer z");
+  if (w && (e = pl->w.ensure(npx * L * sizeof(cos_cf32))) != hipSuccess) return fail_hip(e, "device buffer w");
+  if (P && (e = pl->P.ensure(npx * sizeof(float))) != hipSuccess) return fail_hip(e, "device buffer P");
+  if (needR && (e = pl->R.ensure(npx * L * L * sizeof(cos_cf32))) != hipSuccess)
+    return fail_hip(e, "device buffer R");
+  if ((e = hipMemcpy(pl->x.p, x, xb, hipMemcpyHostToDevice)) != hipSuccess) return fail_hip(e, "copy x");
+
+  cos_capon_desc ds;
+  std::memset(&ds, 0, sizeof(ds));
+  ds.Nx = Nx; ds.Nz = Nz; ds.Mstride = M; ds.M = M; ds.L = L; ds.off = 0; ds.Yavg = Yavg; ds.d = d;
+  ds.precision = COS_PREC_FP32;
+  if (Nb == 0) {
+    int r = cos_capon_run(pl, &ds, pl->x.as<cos_cf32>(), P ? pl->P.as<float>() : nullptr,
+                          z ? pl->z.as<cos_cf32>() : nullptr, w ? pl->w.as<cos_cf32>() : nullptr, L,
+                          R ? pl->R.as<cos_cf32>() : nullptr, nullptr, nullptr);
+    if (r) return r;
+  } else {
+    // Reference default beamspace: element-space loaded S_L, then B S_L B^H (Capon.cpp:285-402).
+    cosk::CaponLaunch a = base_launch(ds);
+    a.R = pl->R.as<float2>();
+    a.Rstride = L * L;
+    a.cov_only = 1;
+    const char* why = nullptr;
+    if ((e = cosk::launch_capon_f32(a, L, M, nullptr, &why)) != hipSuccess)
+      return why ? fail(e, why) : fail_hip(e, "covariance kernel");
+    std::vector<float> Bh = butler_host(Nb, L);
+    if ((e = pl->B.ensure(Bh.size() * sizeof(float))) != hipSuccess) return fail_hip(e, "device buffer B");
+    if ((e = hipMemcpy(pl->B.p, Bh.data(), Bh.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail_hip(e, "copy B");
+    e = cosk::beamspace_matrix(pl->R.as<float2>(), pl->B.as<float2>(), pl->x.as<float2>(), L, Nb, M, Yavg, Nx,
+                               Nz, w ? pl->w.as<float2>() : nullptr, z ? pl->z.as<float2>() : nullptr,
+                               P ? pl->P.as<float>() : nullptr, nullptr);
+    if (e != hipSuccess) return fail_hip(e, "beamspace kernel");
+  }
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return fail_hip(e, "kernel execution");
+  if (z && (e = hipMemcpy(z, pl->z.p, npx * sizeof(cos_cf32), hipMemcpyDeviceToHost)) != hipSuccess)
+    return fail_hip(e, "copy z");
+  if (w && (e = hipMemcpy(w, pl->w.p, npx * L * sizeof(cos_cf32), hipMemcpyDeviceToHost)) != hipSuccess)
+    return fail_hip(e, "copy w");
+  if (P && (e = hipMemcpy(P, pl->P.p, npx * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)
+    return fail_hip(e, "copy P");
+  if (R && (e = hipMemcpy(R, pl->R.p, npx * L * L * sizeof(cos_cf32), hipMemcpyDeviceToHost)) != hipSuccess)
+    return fail_hip(e, "copy R");
+  return 0;
+}
+
+int cos_capon_get_capon(cos_cf32* z, cos_cf32* w, cos_cf32* R, const cos_cf32* x, float d, int L, int Yavg,
+                        int M, int Nx, int Nz, int Nb) {
+  return cos_capon_get_capon_p(z, w, R, nullptr, x, d, L, Yavg, M, Nx, Nz, Nb);
+}
+
+int cos_buildr_get_r(cos_cf32* R, const cos_cf32* x, float d, int L, int Yavg, int M, int Nx, int Nz, int Nb,
+                     int R_on_gpu, int x_on_gpu) {
+  g_last_error.clear();
+  // Argument checks of RealTimeCapon/BuildR.cpp:37-48.
+  if (Nx < 1 || Nz < 1 || L < 1 || M < 1 || Yavg < 0 || Nb < 0)
+    return fail(hipErrorInvalidValue, "Nx, Nz, L, M, Yavg or Nb were <= 0");
+  if (L > M) return fail(hipErrorInvalidValue, "L > M");
+  if (2 * Yavg >= Nz) return fail(hipErrorInvalidValue, "Nz <= Yavg");
+  if (!R || !x) return fail(hipErrorInvalidValue, "R and x must be non-NULL");
+  if (Nb == 0 && L > 64) return fail(hipErrorInvalidValue, "L > 64 is not supported");
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  cos_capon_plan* pl = default_plan();
+  const int K = M - L + 1;
+  const size_t npx = (size_t)Nx * (Nz - 2 * Yavg);
+  const int Ld = Nb > 0 ? Nb : L;
+  const size_t Rb = npx * Ld * Ld * sizeof(cos_cf32);
+  const size_t xb = (size_t)Nx * Nz * (Nb > 0 ? (size_t)K * Nb : (size_t)M) * sizeof(cos_cf32);
+  hipError_t e;
+  const cos_cf32* xd = x;
+  cos_cf32* Rd = R;
+  if (!x_on_gpu) {
+    if ((e = pl->x.ensure(xb)) != hipSuccess) return fail_hip(e, "device buffer x");
+    if ((e = hipMemcpy(pl->x.p, x, xb, hipMemcpyHostToDevice)) != hipSuccess) return fail_hip(e, "copy x");
+    xd = pl->x.as<cos_cf32>();
+  }
+  if (!R_on_gpu) {
+    if ((e = pl->R.ensure(Rb)) != hipSuccess) return fail_hip(e, "device buffer R");
+    Rd = pl->R.as<cos_cf32>();
+  }
+  if (Nb == 0) {
+    cos_capon_desc ds;
+    std::memset(&ds, 0, sizeof(ds));
+    ds.Nx = Nx; ds.Nz = Nz; ds.Mstride = M; ds.M = M; ds.L = L; ds.Yavg = Yavg; ds.d = d;
+    cosk::CaponLaunch a = base_launch(ds);
+    a.x = (const float2*)xd;
+    a.R = (float2*)Rd;
+    a.Rstride = L * L;
+    a.cov_only = 1;
+    const char* why = nullptr;
+    if ((e = cosk::launch_capon_f32(a, L, M, nullptr, &why)) != hipSuccess)
+      return why ? fail(e, why) : fail_hip(e, "covariance kernel");
+  } else {
+    if ((e = cosk::buildr_beamspace((const float2*)xd, (float2*)Rd, d, Nb, K, Yavg, Nx, Nz, nullptr)) != hipSuccess)
+      return fail_hip(e, "beamspace covariance kernel");
+  }
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return fail_hip(e, "kernel execution");
+  if (!R_on_gpu && (e = hipMemcpy(R, Rd, Rb, hipMemcpyDeviceToHost)) != hipSuccess) return fail_hip(e, "copy R");
+  return 0;
+}
+
+int cos_solver_solve(cos_cf32* x, const cos_cf32* A, const cos_cf32* b, int N, int batch, int x_on_gpu,
+                     int A_on_gpu, int b_on_gpu) {
+  g_last_error.clear();
+  // Solver::solve accepts 1 <= N <= 72 (RealTimeCapon/Solver.cpp:34-42).
+  if (N < 1 || N > 72 || batch < 1) return fail(hipErrorInvalidValue, "need 1 <= N <= 72 and batch >= 1");
+  if (!x || !A || !b) return fail(hipErrorInvalidValue, "x, A and b must be non-NULL");
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  cos_capon_plan* pl = default_plan();
+  const size_t Ab = (size_t)batch * N * N * sizeof(cos_cf32), vb = (size_t)batch * N * sizeof(cos_cf32);
+  hipError_t e;
+  const cos_cf32* Ad = A;
+  const cos_cf32* bd = b;
+  cos_cf32* xd = x;
+  if (!A_on_gpu) {
+    if ((e = pl->R.ensure(Ab)) != hipSuccess) return fail_hip(e, "device buffer A");
+    if ((e = hipMemcpy(pl->R.p, A, Ab, hipMemcpyHostToDevice)) != hipSuccess) return fail_hip(e, "copy A");
+    Ad = pl->R.as<cos_cf32>();
+  }
+  if (!b_on_gpu) {
+    if ((e = pl->z.ensure(vb)) != hipSuccess) return fail_hip(e, "device buffer b");
+    if ((e = hipMemcpy(pl->z.p, b, vb, hipMemcpyHostToDevice)) != hipSuccess) return fail_hip(e, "copy b");
+    bd = pl->z.as<cos_cf32>();
+  }
+  if (!x_on_gpu) {
+    if ((e = pl->w.ensure(vb)) != hipSuccess) return fail_hip(e, "device buffer x");
+    xd = pl->w.as<cos_cf32>();
+  }
+  if ((e = cosk::solve_lu((const float2*)Ad, (const float2*)bd, (float2*)xd, N, batch, nullptr)) != hipSuccess)
+    return fail_hip(e, "solver kernel");
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return fail_hip(e, "kernel execution");
+  if (!x_on_gpu && (e = hipMemcpy(x, xd, vb, hipMemcpyDeviceToHost)) != hipSuccess) return fail_hip(e, "copy x");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,415 @@
+// Fused per-pixel Capon/MVDR kernel for gfx950: covariance build -> diagonal
+// loading -> Cholesky -> forward/back solves -> power, amplitude and weights, all
+// in registers of one lane group per pixel.
+//
+// Semantics follow the reference (paths relative to its root):
+//   covariance   framework/beamformer/capon/RealTimeCapon/buildR_kernel.cu:45-161
+//                S[r][c] = sum_t sum_{k<K} x_t[k+r] conj(x_t[k+c]), unnormalised,
+//                loading d*tr(S)/L on the diagonal (tr = sum of real parts)
+//   solve        RealTimeCapon/Solver.cpp:23-116 (here: Cholesky of S_L, a = 1)
+//   outputs      RealTimeCapon/capon_kernel.cu:338-427: w = y / sum(y),
+//                z = (1/K) sum_l conj(w_l) sum_k x_c[k+l]
+//   power        P = 1 / (a^H R_L^-1 a) for R_L = S_L / (K(2Y+1))
+//                (framework/beamformer/capon/getCaponC.pyx:240-289, intended zPow)
+//
+// Layout in registers.  A group of G lanes (G = 16: one DPP row, four pixels per
+// wave; G = 64: one pixel per wave) owns one pixel.  Lane g owns the matrix columns
+// c_s = g + G*s, s < CPL, so L <= LMAX = G*CPL.  Slot s keeps rows 0 .. G(s+1)-1 of
+// its column, which covers the upper triangle (row <= column) for every lane.
+// Systems with L < LMAX are padded with an identity block, so all control flow is
+// uniform and per-pixel L, M and d may differ inside one wave.
+//
+// Covariance (FP32 path): the first row by direct sums, then the displacement
+// recursion S[r][c] = S[r-1][c-1] + sum_t (x_t[K-1+r] conj(x_t[K-1+c]) -
+// x_t[r-1] conj(x_t[c-1])), one DPP rotate per row, the same identity the
+// reference's buildR_kernel uses along diagonals (buildR_kernel.cu:116-137).
+// The FP64 path ("plain" per-pixel kernel, also the fallback for ill-conditioned
+// pixels) uses direct sums for every element.
+//
+// Cholesky: right-looking U^H U on the upper triangle; row k is broadcast lane by
+// lane with DPP row_newbcast (G=16) or v_readlane (G=64).  The forward solves with
+// U^H for the two right-hand sides a and g are column-oriented and local; their
+// running dot products give a^H R^-1 a = |U^-H a|^2 and y^H g = (U^-H a)^H (U^-H g)
+// without a reduction.  The back solve U y = U^-H a (only needed for the weights)
+// is row-oriented and uses one group reduction per row.
+#pragma once
+#include "cos_device.h"
+#include "capon_kernels.h"
+
+namespace cosk {
+
+template <typename T>
+__device__ __forceinline__ cplx<T> ldx(const float2* p) {
+  float2 v = *p;
+  return {(T)v.x, (T)v.y};
+}
+
+template <int G, int S, int R>
+__device__ __forceinline__ constexpr int slot() { return G * S * (S + 1) / 2 + R; }
+
+template <typename T> __device__ __forceinline__ T big_value();
+template <> __device__ __forceinline__ float big_value<float>() { return 3.0e38f; }
+template <> __device__ __forceinline__ double big_value<double>() { return 1.0e300; }
+template <typename T> __device__ __forceinline__ T tiny_value();
+template <> __device__ __forceinline__ float tiny_value<float>() { return 1.0e-30f; }
+template <> __device__ __forceinline__ double tiny_value<double>() { return 1.0e-300; }
+
+// T_: scalar type (float or double); G, CPL: lane group and columns per lane;
+// TM: compile-time bound on the window length 2Y+1 (0 = runtime, reload from LDS);
+// DIRECT: direct covariance sums instead of the displacement recursion.
+// Occupancy target (waves per SIMD) the register allocator is held to.
+template <typename T, int G, int CPL> struct OccHint {
+  static constexpr int value = sizeof(T) == 4 ? (G == 16 ? (CPL == 1 ? 4 : 2) : 1) : (CPL == 1 ? 2 : 1);
+};
+
+template <typename T, int G, int CPL, int TM, bool DIRECT>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
+                          amdgpu_waves_per_eu(OccHint<T, G, CPL>::value)))
+void capon_kernel(const CaponLaunch a, const int MP) {
+  using Gp = Grp<G>;
+  constexpr int NG = Gp::NG, LMAX = G * CPL, NS = G * CPL * (CPL + 1) / 2;
+  constexpr int TMR = TM > 0 ? TM : 1;
+  extern __shared__ float2 xs_all[];
+
+  const int g = Gp::lane(), q = Gp::group();
+  const int Y = a.Y, TW = 2 * Y + 1, Nzp = a.Nz - 2 * Y;
+  float2* xs = xs_all + q * TW * MP;
+
+  const int npix = a.pcount ? min(*a.pcount, a.npix) : a.npix;
+  const int gid0 = blockIdx.x * NG;
+  if (gid0 >= npix) return;  // whole wave idle (uniform)
+  const int gid = gid0 + q;
+  const bool active = gid < npix;
+  const int gsel = active ? gid : gid0;
+  const int p = a.plist ? a.plist[gsel] : gsel;
+  const int ix = p / Nzp, j = p - ix * Nzp, crow = j + Y;
+  const int L = a.rowL ? a.rowL[crow] : a.L;
+  const int M = a.rowM ? a.rowM[crow] : a.M;
+  const int off = a.rowOff ? a.rowOff[crow] : a.off;
+  const float dload = a.rowD ? a.rowD[crow] : a.d;
+  const int K = M - L + 1;
+
+  // Stage the 2Y+1 snapshots of the window, zero padded to MP elements.
+  for (int t = 0; t < TW; ++t) {
+    const float2* src = a.x + ((size_t)ix * a.Nz + (j + t)) * a.Mstride + off;
+    for (int m = g; m < MP; m += G) xs[t * MP + m] = (m < M) ? src[m] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+
+  int cs[CPL];
+  static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value; cs[S] = g + G * S; });
+
+  cplx<T> A[NS];
+
+  // ---------------- covariance S (unnormalised) ----------------
+  if constexpr (DIRECT) {
+    static_for<0, NS>([&](auto I_) {
+    constexpr int I = decltype(I_)::value; A[I] = {T(0), T(0)}; });
+    for (int t = 0; t < TW; ++t) {
+      const float2* xr = xs + t * MP;
+      for (int k = 0; k < K; ++k) {
+        cplx<T> b[CPL];
+        static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value; b[S] = ldx<T>(xr + k + cs[S]); });
+        static_for<0, LMAX>([&](auto R_) {
+    constexpr int R = decltype(R_)::value;
+          const cplx<T> u = ldx<T>(xr + k + R);
+          static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+            if constexpr (R < G * (S + 1)) add_mul_conj(A[slot<G, S, R>()], u, b[S]);
+          });
+        });
+      }
+    }
+  } else {
+    cplx<T> s0[CPL];
+    static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value; s0[S] = {T(0), T(0)}; });
+    for (int t = 0; t < TW; ++t) {
+      const float2* xr = xs + t * MP;
+      for (int k = 0; k < K; ++k) {
+        const cplx<T> u = ldx<T>(xr + k);
+        static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value; add_mul_conj(s0[S], u, ldx<T>(xr + k + cs[S])); });
+      }
+    }
+    static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value; A[slot<G, S, 0>()] = s0[S]; });
+
+    cplx<T> al[TMR][CPL], be[TMR][CPL];
+    if constexpr (TM > 0) {
+      static_for<0, TM>([&](auto t_) {
+    constexpr int t = decltype(t_)::value;
+        static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+          if (t < TW) {
+            al[t][S] = ldx<T>(xs + t * MP + K - 1 + cs[S]);
+            be[t][S] = ldx<T>(xs + t * MP + max(cs[S] - 1, 0));
+          } else {
+            al[t][S] = {T(0), T(0)};
+            be[t][S] = {T(0), T(0)};
+          }
+        });
+      });
+    }
+    static_for<1, LMAX>([&](auto R_) {
+    constexpr int R = decltype(R_)::value;
+      cplx<T> nv[CPL];
+      static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+        if constexpr (R < G * (S + 1)) {
+          cplx<T> own = {Gp::ror1(A[slot<G, S, R - 1>()].x), Gp::ror1(A[slot<G, S, R - 1>()].y)};
+          if constexpr (S > 0 && R - 1 < G * S) {
+            const cplx<T> lo = {Gp::ror1(A[slot<G, S - 1, R - 1>()].x),
+                                Gp::ror1(A[slot<G, S - 1, R - 1>()].y)};
+            if (g == 0) own = lo;
+          }
+          nv[S] = own;
+        }
+      });
+      auto corr = [&](int t, const cplx<T>* alt, const cplx<T>* bet) {
+        const cplx<T> u1 = ldx<T>(xs + t * MP + K - 1 + R);
+        const cplx<T> u0 = ldx<T>(xs + t * MP + R - 1);
+        static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+          if constexpr (R < G * (S + 1)) {
+            add_mul_conj(nv[S], u1, alt[S]);
+            sub_mul_conj(nv[S], u0, bet[S]);
+          }
+        });
+      };
+      if constexpr (TM > 0) {
+        static_for<0, TM>([&](auto t_) {
+    constexpr int t = decltype(t_)::value;
+          if (t < TW) corr(t, al[t], be[t]);
+        });
+      } else {
+        for (int t = 0; t < TW; ++t) {
+          cplx<T> alt[CPL], bet[CPL];
+          static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+            alt[S] = ldx<T>(xs + t * MP + K - 1 + cs[S]);
+            bet[S] = ldx<T>(xs + t * MP + max(cs[S] - 1, 0));
+          });
+          corr(t, alt, bet);
+        }
+      }
+      static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+        if constexpr (R < G * (S + 1)) A[slot<G, S, R>()] = nv[S];
+      });
+    });
+  }
+
+  // ---------------- trace, loading, padding ----------------
+  T trp = T(0);
+  static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+    static_for<0, G * (S + 1)>([&](auto R_) {
+    constexpr int R = decltype(R_)::value;
+      trp += (cs[S] == R && R < L) ? A[slot<G, S, R>()].x : T(0);
+    });
+  });
+  const T trS = Gp::sum(trp);
+  const T delta = (T)dload * trS / (T)L;
+  if (L < LMAX) {
+    static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+      static_for<0, G * (S + 1)>([&](auto R_) {
+    constexpr int R = decltype(R_)::value;
+        if (cs[S] >= L) A[slot<G, S, R>()] = {(cs[S] == R) ? T(1) : T(0), T(0)};
+      });
+    });
+  }
+
+  // Optional materialised R in the reference layout: mem[r*L+c] = conj(S_L[r][c]).
+  if (a.R && active) {
+    float2* Rp = a.R + (size_t)p * a.Rstride;
+    static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+      static_for<0, G * (S + 1)>([&](auto R_) {
+    constexpr int R = decltype(R_)::value;
+        const int cc = cs[S];
+        if (cc < L && R <= cc) {
+          cplx<T> v = A[slot<G, S, R>()];
+          if (R == cc) v.x += delta;
+          Rp[R * L + cc] = make_float2((float)v.x, (float)-v.y);
+          if (R < cc) Rp[cc * L + R] = make_float2((float)v.x, (float)v.y);
+        }
+      });
+    });
+  }
+
+  if (a.cov_only) return;
+
+  // ---------------- Cholesky U^H U = S_L ----------------
+  T pmin = big_value<T>(), pmax = T(0);
+  bool notpd = false;
+  static_for<0, LMAX>([&](auto Kk_) {
+    constexpr int Kk = decltype(Kk_)::value;
+    constexpr int k = Kk, sk = k / G, lk = k % G;
+    T piv = Gp::template bcast<lk>(A[slot<G, sk, k>()].x) + ((k < L) ? delta : T(0));
+    const bool ok = piv > tiny_value<T>();
+    notpd |= !ok;
+    piv = ok ? piv : tiny_value<T>();
+    if (k < L) {
+      pmin = min(pmin, piv);
+      pmax = max(pmax, piv);
+    }
+    const T inv = rsqrt_t<T>(piv);
+    static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+      if constexpr (k < G * (S + 1)) {
+        A[slot<G, S, k>()].x *= inv;
+        A[slot<G, S, k>()].y *= inv;
+      }
+    });
+    if (g == lk) A[slot<G, sk, k>()] = {piv * inv, inv};
+    static_for<k + 1, LMAX>([&](auto Ii_) {
+    constexpr int Ii = decltype(Ii_)::value;
+      constexpr int i = Ii, si = i / G, li = i % G;
+      const cplx<T> u = bcastc<G, li>(A[slot<G, si, k>()]);
+      static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+        if constexpr (i < G * (S + 1)) sub_conj_mul(A[slot<G, S, i>()], u, A[slot<G, S, k>()]);
+      });
+    });
+  });
+
+  const T norm = (T)K * (T)TW;
+  const int nrhs = a.nsteer > 0 ? a.nsteer : 1;
+  const float2* xc = xs + Y * MP;
+  const T invK = T(1) / (T)K;
+
+  for (int ia = 0; ia < nrhs; ++ia) {
+    const T sn = a.nsteer > 0 ? (T)a.steer_sin[ia] : T(0);
+    // right-hand sides a (ones or steering vector) and g (subarray mean of the centre row)
+    cplx<T> b1[CPL], b2[CPL], zA[CPL];
+    static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+      const int cc = cs[S];
+      zA[S] = {T(0), T(0)};
+      cplx<T> gs = {T(0), T(0)};
+      if (a.nsteer > 0) {
+        float sv, cv;
+        sincospif((float)(sn * (T)cc), &sv, &cv);
+        b1[S] = {(T)cv, (T)sv};
+        for (int k = 0; k < K; ++k) {
+          float s2, c2;
+          sincospif(-(float)(sn * (T)k), &s2, &c2);
+          add_mul(gs, ldx<T>(xc + k + cc), cplx<T>{(T)c2, (T)s2});
+        }
+      } else {
+        b1[S] = {T(1), T(0)};
+        for (int k = 0; k < K; ++k) {
+          const cplx<T> v = ldx<T>(xc + k + cc);
+          gs.x += v.x;
+          gs.y += v.y;
+        }
+      }
+      if (cc >= L) {
+        b1[S] = {T(0), T(0)};
+        gs = {T(0), T(0)};
+      }
+      b2[S] = {gs.x * invK, gs.y * invK};
+    });
+
+    T accP = T(0);
+    cplx<T> accZ = {T(0), T(0)};
+    static_for<0, LMAX>([&](auto Kk_) {
+    constexpr int Kk = decltype(Kk_)::value;
+      constexpr int k = Kk, sk = k / G, lk = k % G;
+      const T inv = Gp::template bcast<lk>(A[slot<G, sk, k>()].y);
+      cplx<T> v1 = bcastc<G, lk>(b1[sk]);
+      cplx<T> v2 = bcastc<G, lk>(b2[sk]);
+      v1.x *= inv; v1.y *= inv;
+      v2.x *= inv; v2.y *= inv;
+      accP = fma(v1.x, v1.x, fma(v1.y, v1.y, accP));
+      add_conj_mul(accZ, v1, v2);
+      if (g == lk) zA[sk] = v1;
+      static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+        if constexpr (k < G * (S + 1)) {
+          sub_conj_mul(b1[S], A[slot<G, S, k>()], v1);
+          sub_conj_mul(b2[S], A[slot<G, S, k>()], v2);
+        }
+      });
+    });
+
+    const size_t po = (size_t)p * nrhs + ia;
+    if (active && g == 0) {
+      if (a.P) a.P[po] = (float)(T(1) / (norm * accP));
+      if (a.z) a.z[po] = make_float2((float)(accZ.x / accP), (float)(accZ.y / accP));
+    }
+
+    if (a.w) {
+      // back solve U y = U^-H a, one row per step, reduction over the group
+      cplx<T> y[CPL];
+      static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value; y[S] = {T(0), T(0)}; });
+      static_for<LMAX - 1, -1, -1>([&](auto Cc_) {
+    constexpr int Cc = decltype(Cc_)::value;
+        constexpr int c = Cc, sc = c / G, lc = c % G;
+        cplx<T> part = {T(0), T(0)};
+        static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+          if constexpr (c < G * (S + 1)) {
+            cplx<T> pr = {T(0), T(0)};
+            add_mul(pr, A[slot<G, S, c>()], y[S]);
+            const bool use = cs[S] > c;
+            part.x += use ? pr.x : T(0);
+            part.y += use ? pr.y : T(0);
+          }
+        });
+        part.x = Gp::sum(part.x);
+        part.y = Gp::sum(part.y);
+        const T inv = A[slot<G, sc, c>()].y;
+        const cplx<T> val = {(zA[sc].x - part.x) * inv, (zA[sc].y - part.y) * inv};
+        if (g == lc) y[sc] = val;
+      });
+      if (active) {
+        const T sc = T(1) / accP;
+        static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+          const int cc = cs[S];
+          if (cc < L) {
+            cplx<T> wv = {y[S].x * sc, y[S].y * sc};
+            if (a.nsteer > 0) {
+              float sv, cv;
+              sincospif(-(float)(sn * (T)cc), &sv, &cv);
+              cplx<T> r = {T(0), T(0)};
+              add_mul(r, wv, cplx<T>{(T)cv, (T)sv});
+              wv = r;
+            }
+            a.w[po * a.wstride + cc] = make_float2((float)wv.x, (float)wv.y);
+          }
+        });
+      }
+    }
+  }
+
+  if (a.flags && active && g == 0) {
+    uint8_t f = 0;
+    if ((T(1) + (T)L / (T)dload) > (T)a.cond_thresh) f |= 1;
+    if (notpd) f |= 2;
+    a.flags[p] = f;
+  }
+  (void)pmin;
+  (void)pmax;
+}
+
+template <typename T, int G, int CPL, int TM, bool DIRECT>
+hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
+  constexpr int NG = 64 / G, LMAX = G * CPL;
+  const int MP = ((Mmax + LMAX + 3) / 4) * 4;
+  const int TW = 2 * a.Y + 1;
+  const size_t lds = (size_t)NG * TW * MP * sizeof(float2);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  const int blocks = (a.npix + NG - 1) / NG;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT>), dim3(blocks), dim3(64), lds, s, a, MP);
+  return hipGetLastError();
+}
+
+}  // namespace cosk

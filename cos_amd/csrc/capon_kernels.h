@@ -1,0 +1,42 @@
+// Internal (non-ABI) launch interface shared by the kernel and API translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cosk {
+
+// Everything one launch of the per-pixel Capon kernel needs.  Sizes and pointers
+// are plain; all pointers are device pointers.  Pixel p = ix*(Nz-2Y) + j is the
+// reference's output order (RTC/capon_kernel.cu:364-391); its centre row is c = j+Y.
+struct CaponLaunch {
+  const float2* x;     // cube [Nx][Nz][Mstride] (complex float interleaved)
+  int Nx, Nz, Mstride, Y;
+  int L, M, off;       // uniform subarray length / active elements / first element
+  float d;             // uniform relative diagonal loading
+  const int* rowL;     // optional per-centre-row tables, length Nz (override L/M/off/d)
+  const int* rowM;
+  const int* rowOff;
+  const float* rowD;
+  const int* plist;    // optional pixel list (values p); null = all pixels
+  const int* pcount;   // optional device count for plist (overrides npix when set)
+  int npix;            // number of pixels (list capacity when pcount is set)
+  float* P;            // [npix_total] power 1/(a^H R_L^-1 a), R normalised by K(2Y+1)
+  float2* z;           // [npix_total] amplitude w^H g
+  float2* w;           // [npix_total][wstride] weights (first L entries)
+  int wstride;
+  float2* R;           // [npix_total][Rstride] loaded unnormalised S_L, conj-stored
+  int Rstride;
+  uint8_t* flags;      // [npix_total] bit0: cond bound > cond_thresh, bit1: non-PD pivot
+  float cond_thresh;
+  int refine;          // FP32 path: one step of iterative refinement of y
+  int cov_only;        // stop after the (optional) R output: BuildR::getR
+  // steering (optional): nsteer sines; outputs P/z are [p][nsteer], w is [p][nsteer][wstride]
+  const float* steer_sin;
+  int nsteer;
+};
+
+// Returns hipSuccess or an error; `why` receives a static message on argument errors.
+hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
+hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
+
+}  // namespace cosk

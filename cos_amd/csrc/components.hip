@@ -1,0 +1,302 @@
+// Component kernels around the fused Capon kernel: fallback compaction, the batched
+// general solver behind Solver::solve, the beamspace covariance behind
+// BuildR::getR(Nb > 0), and the reference's matrix-beamspace output path.
+#include "cos_device.h"
+#include "components.h"
+
+namespace cosk {
+
+__global__ void fill_i32_kernel(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Pixels with any flag bit go to the FP64 list (order is irrelevant: pixels are
+// independent and each writes only its own outputs).
+__global__ void compact_flags_kernel(const uint8_t* flags, int n, int* list, int* count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flags[i] != 0) {
+    int k = atomicAdd(count, 1);
+    list[k] = i;
+  }
+}
+
+hipError_t fill_i32(int* p, int n, int v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t compact_flags(const uint8_t* flags, int n, int* list, int* count, hipStream_t s) {
+  hipError_t e = fill_i32(count, 1, 0, s);
+  if (e != hipSuccess || n <= 0) return e;
+  hipLaunchKernelGGL(compact_flags_kernel, dim3((n + 255) / 256), dim3(256), 0, s, flags, n, list, count);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Batched general complex solve A x = b, A column-major (A[i + N*j] = A_ij), one
+// 64-thread block per system, matrix and right-hand side in LDS, partial pivoting on
+// |re| + |im| (the pivot measure of RealTimeCapon/operations.h:183-207).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void solve_lu_kernel(const float2* A, const float2* b, float2* x,
+                                                      int N, int batch) {
+  extern __shared__ float2 sm[];
+  const int sys = blockIdx.x;
+  if (sys >= batch) return;
+  const int NP = N + 1;           // row stride of the augmented matrix [N][N+1]
+  float2* M = sm;
+  __shared__ int piv;
+  const float2* As = A + (size_t)sys * N * N;
+  for (int e = threadIdx.x; e < N * N; e += 64) {
+    int i = e % N, jj = e / N;
+    M[i * NP + jj] = As[e];
+  }
+  for (int i = threadIdx.x; i < N; i += 64) M[i * NP + N] = b[(size_t)sys * N + i];
+  __syncthreads();
+  for (int k = 0; k < N; ++k) {
+    if (threadIdx.x == 0) {
+      int best = k;
+      float bv = -1.f;
+      for (int i = k; i < N; ++i) {
+        float v = fabsf(M[i * NP + k].x) + fabsf(M[i * NP + k].y);
+        if (v > bv) { bv = v; best = i; }
+      }
+      piv = best;
+    }
+    __syncthreads();
+    const int pr = piv;
+    if (pr != k) {
+      for (int jj = threadIdx.x; jj <= N; jj += 64) {
+        float2 t = M[k * NP + jj];
+        M[k * NP + jj] = M[pr * NP + jj];
+        M[pr * NP + jj] = t;
+      }
+    }
+    __syncthreads();
+    const float2 pv = M[k * NP + k];
+    const float den = pv.x * pv.x + pv.y * pv.y;
+    const float2 ipv = make_float2(pv.x / den, -pv.y / den);
+    // eliminate below the pivot; thread handles rows i
+    for (int i = k + 1 + threadIdx.x; i < N; i += 64) {
+      float2 a = M[i * NP + k];
+      float2 f = make_float2(a.x * ipv.x - a.y * ipv.y, a.x * ipv.y + a.y * ipv.x);
+      for (int jj = k + 1; jj <= N; ++jj) {
+        float2 u = M[k * NP + jj];
+        float2 v = M[i * NP + jj];
+        v.x -= f.x * u.x - f.y * u.y;
+        v.y -= f.x * u.y + f.y * u.x;
+        M[i * NP + jj] = v;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int i = N - 1; i >= 0; --i) {
+      float2 s = M[i * NP + N];
+      for (int jj = i + 1; jj < N; ++jj) {
+        float2 u = M[i * NP + jj];
+        float2 v = M[jj * NP + N];
+        s.x -= u.x * v.x - u.y * v.y;
+        s.y -= u.x * v.y + u.y * v.x;
+      }
+      float2 d = M[i * NP + i];
+      float den = d.x * d.x + d.y * d.y;
+      float2 r = make_float2((s.x * d.x + s.y * d.y) / den, (s.y * d.x - s.x * d.y) / den);
+      M[i * NP + N] = r;
+      x[(size_t)sys * N + i] = r;
+    }
+  }
+}
+
+hipError_t solve_lu(const float2* A, const float2* b, float2* x, int N, int batch, hipStream_t s) {
+  if (batch <= 0) return hipSuccess;
+  size_t lds = (size_t)N * (N + 1) * sizeof(float2);
+  hipLaunchKernelGGL(solve_lu_kernel, dim3(batch), dim3(64), lds, s, A, b, x, N, batch);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// BuildR with Nb > 0: x_bs [Nx][Nz][K][Nb]; R[r][c] over k and the window, loading
+// d*trace/Nb, stored conjugated like buildR_full_kernel (buildR_kernel.cu:169-251).
+// One thread per (pixel, r, c); the trace is recomputed per thread (Nb is small).
+// ---------------------------------------------------------------------------------
+__global__ void buildr_bs_kernel(const float2* xbs, float2* R, float d, int Nb, int K, int Y,
+                                 int Nx, int Nz) {
+  const int Nzp = Nz - 2 * Y;
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long tot = (long)Nx * Nzp * Nb * Nb;
+  if (tid >= tot) return;
+  const int e = (int)(tid % (Nb * Nb));
+  const long p = tid / (Nb * Nb);
+  const int r = e / Nb, c = e % Nb;
+  const int ix = (int)(p / Nzp), j = (int)(p % Nzp);
+  double sr = 0, si = 0, tr = 0;
+  for (int t = 0; t < 2 * Y + 1; ++t) {
+    const float2* row = xbs + ((size_t)ix * Nz + j + t) * (size_t)K * Nb;
+    for (int k = 0; k < K; ++k) {
+      float2 u = row[k * Nb + r], v = row[k * Nb + c];
+      sr += (double)u.x * v.x + (double)u.y * v.y;
+      si += (double)u.y * v.x - (double)u.x * v.y;
+      for (int l = 0; l < Nb; ++l) {
+        float2 w = row[k * Nb + l];
+        tr += (double)w.x * w.x + (double)w.y * w.y;
+      }
+    }
+  }
+  if (r == c) sr += (double)d * tr / Nb;
+  R[p * Nb * Nb + (size_t)r * Nb + c] = make_float2((float)sr, (float)-si);
+}
+
+hipError_t buildr_beamspace(const float2* xbs, float2* R, float d, int Nb, int K, int Y, int Nx,
+                            int Nz, hipStream_t s) {
+  long tot = (long)Nx * (Nz - 2 * Y) * Nb * Nb;
+  if (tot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(buildr_bs_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, xbs, R,
+                     d, Nb, K, Y, Nx, Nz);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Reference default beamspace path (RealTimeCapon/Capon.cpp:285-402) from the
+// materialised conj-stored S_L: R_bs = B S_L B^H, a_bs = B 1, y_bs = R_bs^-1 a_bs,
+// y = B^H y_bs, then the element-space output stage (capon_kernel.cu:338-427):
+// w = y / sum(y), z = (1/K) sum_l conj(w_l) sum_k x_c[k+l].  P = 1/(norm a_bs^H y_bs).
+// One 64-thread block per pixel; B is [Nb][L] row-major in global memory.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void beamspace_matrix_kernel(const float2* Rg, const float2* B,
+                                                              const float2* x, int L, int Nb, int M,
+                                                              int Y, int Nx, int Nz, float2* w,
+                                                              float2* z, float* P) {
+  extern __shared__ float2 sm[];
+  const int Nzp = Nz - 2 * Y;
+  const int p = blockIdx.x;
+  if (p >= Nx * Nzp) return;
+  float2* S = sm;                 // [L][L] S_L[r][c]
+  float2* Tm = S + L * L;         // [L][Nb] S_L B^H
+  float2* Rb = Tm + L * Nb;       // [Nb][Nb]
+  float2* yb = Rb + Nb * Nb;      // [Nb]
+  float2* ab = yb + Nb;           // [Nb]
+  float2* y = ab + Nb;            // [L]
+  const float2* Rp = Rg + (size_t)p * L * L;
+  for (int e = threadIdx.x; e < L * L; e += 64) {
+    int r = e / L, c = e % L;
+    float2 v = Rp[e];  // conj(S_L[r][c])
+    S[r * L + c] = make_float2(v.x, -v.y);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < L * Nb; e += 64) {
+    int r = e / Nb, bb = e % Nb;
+    double sx = 0, sy = 0;
+    for (int c = 0; c < L; ++c) {  // S[r][c] conj(B[bb][c])
+      float2 s = S[r * L + c], v = B[bb * L + c];
+      sx += (double)s.x * v.x + (double)s.y * v.y;
+      sy += (double)s.y * v.x - (double)s.x * v.y;
+    }
+    Tm[r * Nb + bb] = make_float2((float)sx, (float)sy);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < Nb * Nb; e += 64) {
+    int b1 = e / Nb, b2 = e % Nb;
+    double sx = 0, sy = 0;
+    for (int r = 0; r < L; ++r) {  // B[b1][r] T[r][b2]
+      float2 u = B[b1 * L + r], v = Tm[r * Nb + b2];
+      sx += (double)u.x * v.x - (double)u.y * v.y;
+      sy += (double)u.x * v.y + (double)u.y * v.x;
+    }
+    Rb[b1 * Nb + b2] = make_float2((float)sx, (float)sy);
+  }
+  for (int bb = threadIdx.x; bb < Nb; bb += 64) {
+    double sx = 0, sy = 0;
+    for (int l = 0; l < L; ++l) { sx += B[bb * L + l].x; sy += B[bb * L + l].y; }
+    ab[bb] = make_float2((float)sx, (float)sy);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // Cholesky U^H U = Rb in place (upper triangle), then U^H v = ab, U yb = v; double precision
+    // accumulation, Nb is small.
+    for (int i = 0; i < Nb; ++i) {
+      double dr = Rb[i * Nb + i].x;
+      for (int k = 0; k < i; ++k) {
+        float2 u = Rb[k * Nb + i];
+        dr -= (double)u.x * u.x + (double)u.y * u.y;
+      }
+      double di = sqrt(dr > 0 ? dr : 1e-30);
+      Rb[i * Nb + i] = make_float2((float)di, 0.f);
+      for (int c = i + 1; c < Nb; ++c) {
+        double sx = Rb[i * Nb + c].x, sy = Rb[i * Nb + c].y;
+        for (int k = 0; k < i; ++k) {
+          float2 u = Rb[k * Nb + i], v = Rb[k * Nb + c];  // conj(u) v
+          sx -= (double)u.x * v.x + (double)u.y * v.y;
+          sy -= (double)u.x * v.y - (double)u.y * v.x;
+        }
+        Rb[i * Nb + c] = make_float2((float)(sx / di), (float)(sy / di));
+      }
+    }
+    for (int i = 0; i < Nb; ++i) {
+      double sx = ab[i].x, sy = ab[i].y;
+      for (int k = 0; k < i; ++k) {
+        float2 u = Rb[k * Nb + i], v = yb[k];
+        sx -= (double)u.x * v.x + (double)u.y * v.y;
+        sy -= (double)u.x * v.y - (double)u.y * v.x;
+      }
+      double dd = Rb[i * Nb + i].x;
+      yb[i] = make_float2((float)(sx / dd), (float)(sy / dd));
+    }
+    for (int i = Nb - 1; i >= 0; --i) {
+      double sx = yb[i].x, sy = yb[i].y;
+      for (int k = i + 1; k < Nb; ++k) {
+        float2 u = Rb[i * Nb + k], v = yb[k];
+        sx -= (double)u.x * v.x - (double)u.y * v.y;
+        sy -= (double)u.x * v.y + (double)u.y * v.x;
+      }
+      double dd = Rb[i * Nb + i].x;
+      yb[i] = make_float2((float)(sx / dd), (float)(sy / dd));
+    }
+    double qx = 0;
+    for (int i = 0; i < Nb; ++i) qx += (double)ab[i].x * yb[i].x + (double)ab[i].y * yb[i].y;
+    const int K = M - L + 1;
+    if (P) P[p] = (float)(1.0 / (qx * (double)K * (2 * Y + 1)));
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < L; l += 64) {  // y = B^H yb
+    double sx = 0, sy = 0;
+    for (int bb = 0; bb < Nb; ++bb) {
+      float2 u = B[bb * L + l], v = yb[bb];  // conj(u) v
+      sx += (double)u.x * v.x + (double)u.y * v.y;
+      sy += (double)u.x * v.y - (double)u.y * v.x;
+    }
+    y[l] = make_float2((float)sx, (float)sy);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sx = 0, sy = 0;
+    for (int l = 0; l < L; ++l) { sx += y[l].x; sy += y[l].y; }
+    const double den = sx * sx + sy * sy;
+    const int ix = p / Nzp, j = p % Nzp, K = M - L + 1;
+    const float2* xc = x + ((size_t)ix * Nz + j + Y) * M;
+    double zx = 0, zy = 0;
+    for (int l = 0; l < L; ++l) {
+      double wx = (y[l].x * sx + y[l].y * sy) / den, wy = (y[l].y * sx - y[l].x * sy) / den;
+      if (w) w[(size_t)p * L + l] = make_float2((float)wx, (float)wy);
+      double gx = 0, gy = 0;
+      for (int k = 0; k < K; ++k) { gx += xc[k + l].x; gy += xc[k + l].y; }
+      zx += wx * gx + wy * gy;
+      zy += wx * gy - wy * gx;
+    }
+    if (z) z[p] = make_float2((float)(zx / K), (float)(zy / K));
+  }
+}
+
+hipError_t beamspace_matrix(const float2* R, const float2* B, const float2* x, int L, int Nb, int M,
+                            int Y, int Nx, int Nz, float2* w, float2* z, float* P, hipStream_t s) {
+  const int npx = Nx * (Nz - 2 * Y);
+  if (npx <= 0) return hipSuccess;
+  size_t lds = ((size_t)L * L + (size_t)L * Nb + (size_t)Nb * Nb + 2 * Nb + L) * sizeof(float2);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(beamspace_matrix_kernel, dim3(npx), dim3(64), lds, s, R, B, x, L, Nb, M, Y, Nx,
+                     Nz, w, z, P);
+  return hipGetLastError();
+}
+
+}  // namespace cosk

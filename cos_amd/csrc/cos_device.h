@@ -1,0 +1,144 @@
+// Device-side helpers for the Capon kernels (gfx950, wave64).
+//
+// A "group" is the set of lanes that cooperate on one pixel: 16 lanes (one DPP
+// row, four pixels per wave) or the whole 64-lane wave.  Cross-lane traffic
+// inside a group uses DPP (row_newbcast / row_ror / row_shr) for 16-lane groups
+// and v_readlane / wave_ror for 64-lane groups, so no LDS is spent on
+// broadcasts.  All loops that index register arrays are unrolled at compile
+// time through static_for, which keeps the per-lane matrix columns in VGPRs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+#include <utility>
+
+namespace cosk {
+
+template <int N> using ic = std::integral_constant<int, N>;
+
+template <int B, int E, int S = 1, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr ((S > 0 && B < E) || (S < 0 && B > E)) {
+    f(ic<B>{});
+    static_for<B + S, E, S>(f);
+  }
+}
+
+template <typename T> struct cplx { T x, y; };
+
+template <typename T> __device__ __forceinline__ cplx<T> mk(T a, T b) { return {a, b}; }
+
+// ---- 32-bit DPP / lane primitives ---------------------------------------------------
+template <int CTRL, bool BOUND = false>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, BOUND));
+}
+template <int CTRL, bool BOUND = false>
+__device__ __forceinline__ double dpp(double v) {
+  int2 p = *reinterpret_cast<int2*>(&v);
+  p.x = __builtin_amdgcn_update_dpp(0, p.x, CTRL, 0xF, 0xF, BOUND);
+  p.y = __builtin_amdgcn_update_dpp(0, p.y, CTRL, 0xF, 0xF, BOUND);
+  return *reinterpret_cast<double*>(&p);
+}
+__device__ __forceinline__ float rdlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rdlane(double v, int l) {
+  int2 p = *reinterpret_cast<int2*>(&v);
+  p.x = __builtin_amdgcn_readlane(p.x, l);
+  p.y = __builtin_amdgcn_readlane(p.y, l);
+  return *reinterpret_cast<double*>(&p);
+}
+
+// DPP control words (GFX9 encoding).
+constexpr int DPP_ROW_SHR = 0x110;      // + n, n in 1..15
+constexpr int DPP_ROW_ROR = 0x120;      // + n
+constexpr int DPP_WAVE_ROR1 = 0x13C;
+constexpr int DPP_ROW_NEWBCAST = 0x150; // + lane in row (gfx90a+)
+
+template <int G> struct Grp;
+
+// 16-lane groups: one DPP row per pixel, four pixels per wave.
+template <> struct Grp<16> {
+  static constexpr int NG = 4;
+  __device__ static int lane() { return threadIdx.x & 15; }
+  __device__ static int group() { return (threadIdx.x >> 4) & 3; }
+  template <int S, typename T> __device__ static T bcast(T v) { return dpp<DPP_ROW_NEWBCAST + S>(v); }
+  template <typename T> __device__ static T ror1(T v) { return dpp<DPP_ROW_ROR + 1>(v); }
+  // Sum over the 16 lanes of the row, returned in every lane.
+  template <typename T> __device__ static T sum(T v) {
+    v += dpp<DPP_ROW_SHR + 1, true>(v);
+    v += dpp<DPP_ROW_SHR + 2, true>(v);
+    v += dpp<DPP_ROW_SHR + 4, true>(v);
+    v += dpp<DPP_ROW_SHR + 8, true>(v);
+    return dpp<DPP_ROW_NEWBCAST + 15>(v);
+  }
+};
+
+// Whole-wave groups: one pixel per wave.
+template <> struct Grp<64> {
+  static constexpr int NG = 1;
+  __device__ static int lane() { return threadIdx.x & 63; }
+  __device__ static int group() { return 0; }
+  template <int S, typename T> __device__ static T bcast(T v) { return rdlane(v, S); }
+  template <typename T> __device__ static T ror1(T v) { return dpp<DPP_WAVE_ROR1>(v); }
+  template <typename T> __device__ static T sum(T v) {
+    v += dpp<DPP_ROW_SHR + 1, true>(v);
+    v += dpp<DPP_ROW_SHR + 2, true>(v);
+    v += dpp<DPP_ROW_SHR + 4, true>(v);
+    v += dpp<DPP_ROW_SHR + 8, true>(v);
+    return rdlane(v, 15) + rdlane(v, 31) + rdlane(v, 47) + rdlane(v, 63);
+  }
+};
+
+template <int G, int S, typename T>
+__device__ __forceinline__ cplx<T> bcastc(cplx<T> v) {
+  return {Grp<G>::template bcast<S>(v.x), Grp<G>::template bcast<S>(v.y)};
+}
+
+// a -= conj(u) * v
+template <typename T>
+__device__ __forceinline__ void sub_conj_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
+  a.x = fma(-u.x, v.x, a.x);
+  a.x = fma(-u.y, v.y, a.x);
+  a.y = fma(-u.x, v.y, a.y);
+  a.y = fma(u.y, v.x, a.y);
+}
+// a += u * conj(v)
+template <typename T>
+__device__ __forceinline__ void add_mul_conj(cplx<T>& a, cplx<T> u, cplx<T> v) {
+  a.x = fma(u.x, v.x, a.x);
+  a.x = fma(u.y, v.y, a.x);
+  a.y = fma(u.y, v.x, a.y);
+  a.y = fma(-u.x, v.y, a.y);
+}
+// a -= u * conj(v)
+template <typename T>
+__device__ __forceinline__ void sub_mul_conj(cplx<T>& a, cplx<T> u, cplx<T> v) {
+  a.x = fma(-u.x, v.x, a.x);
+  a.x = fma(-u.y, v.y, a.x);
+  a.y = fma(-u.y, v.x, a.y);
+  a.y = fma(u.x, v.y, a.y);
+}
+// a += u * v
+template <typename T>
+__device__ __forceinline__ void add_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
+  a.x = fma(u.x, v.x, a.x);
+  a.x = fma(-u.y, v.y, a.x);
+  a.y = fma(u.x, v.y, a.y);
+  a.y = fma(u.y, v.x, a.y);
+}
+// a += conj(u) * v
+template <typename T>
+__device__ __forceinline__ void add_conj_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
+  a.x = fma(u.x, v.x, a.x);
+  a.x = fma(u.y, v.y, a.x);
+  a.y = fma(u.x, v.y, a.y);
+  a.y = fma(-u.y, v.x, a.y);
+}
+
+template <typename T> __device__ __forceinline__ T rsqrt_t(T v);
+template <> __device__ __forceinline__ float rsqrt_t<float>(float v) { return 1.0f / __fsqrt_rn(v); }
+template <> __device__ __forceinline__ double rsqrt_t<double>(double v) { return 1.0 / sqrt(v); }
+
+}  // namespace cosk

@@ -1,0 +1,8 @@
+// Explicit instantiations of the fused Capon kernel (one translation unit per
+// variant group so the library builds in parallel).
+#include "capon_fused_impl.h"
+
+namespace cosk {
+template hipError_t launch_one<double, 16, 1, 0, true>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<double, 16, 2, 0, true>(const CaponLaunch&, int, hipStream_t);
+}  // namespace cosk

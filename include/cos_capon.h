@@ -1,0 +1,116 @@
+/*
+ * cos_capon.h -- C ABI of the MI355X Capon/MVDR beamformer library (libcos_capon.so).
+ *
+ * Plain pointers and sizes only.  Complex numbers are {float re, im} interleaved,
+ * layout-identical to the reference's Complex<float> (framework/lib/Complex.h:11-164),
+ * hipFloatComplex and std::complex<float>.
+ *
+ * Data layout of every cube is the reference's x[Nx][Nz][M] (M fastest,
+ * framework/beamformer/capon/RealTimeCapon/buildR_kernel.cu:18,86).  Pixel
+ * p = ix*(Nz-2*Yavg) + j; edge rows are dropped, not padded
+ * (RealTimeCapon/Capon.cpp:90-91).
+ *
+ * Return codes: 0 = success; otherwise a HIP error number (1 = invalid argument,
+ * 2 = out of device memory, others from the HIP runtime).  cos_capon_last_error()
+ * returns a message for the last failure on the calling thread.  No function
+ * writes files or prints.
+ */
+#ifndef COS_CAPON_H
+#define COS_CAPON_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float re, im; } cos_cf32;
+
+/* ----------------------------------------------------------------------------------
+ * Drop-in entry points (host pointers; the library copies to and from the GPU).
+ * ---------------------------------------------------------------------------------- */
+
+/* Replaces Capon::getCapon (RealTimeCapon/ICapon.h:20-33, RealTimeCapon/Capon.cpp:73-152).
+ * z [Nx][Nz-2Y] amplitude, w [Nx][Nz-2Y][L] weights (element space also when Nb > 0),
+ * R [Nx][Nz-2Y][L][L] the loaded, unnormalised covariance stored conjugated (= column
+ * major), as the reference copies back with COPY_R_DEVICE_HOST (Capon.cpp:9,255-260);
+ * R may be NULL to skip it.  d is the relative diagonal loading (d*trace/L).
+ * Nb > 0 selects the reference's default matrix beamspace path with Nb Butler beams
+ * (Capon.cpp:285-402).  Requirements as the reference: 1 <= L <= M, Yavg >= 0,
+ * 2*Yavg < Nz, 0 <= Nb <= L; in addition L <= 64. */
+int cos_capon_get_capon(cos_cf32* z, cos_cf32* w, cos_cf32* R, const cos_cf32* x, float d,
+                        int L, int Yavg, int M, int Nx, int Nz, int Nb);
+
+/* As cos_capon_get_capon, plus the power image P [Nx][Nz-2Y] = 1/(a^H R_L^-1 a) with
+ * R_L = S_L/(K(2Y+1)); no reference counterpart (the intended zPow of
+ * framework/beamformer/capon/getCaponC.pyx:285-289).  Any output may be NULL. */
+int cos_capon_get_capon_p(cos_cf32* z, cos_cf32* w, cos_cf32* R, float* P, const cos_cf32* x,
+                          float d, int L, int Yavg, int M, int Nx, int Nz, int Nb);
+
+/* Replaces BuildR::getR (RealTimeCapon/IBuildR.h:51-64, RealTimeCapon/BuildR.cpp:21-106).
+ * Nb == 0: R [Nx][Nz-2Y][L][L] loaded unnormalised conj-stored covariance of x [Nx][Nz][M].
+ * Nb > 0: x holds beamspace-transformed subarrays [Nx][Nz][K][Nb] (K = M-L+1) and R is
+ * [Nx][Nz-2Y][Nb][Nb] with loading d*trace/Nb (BuildR.cpp:82-86, buildR_kernel.cu:169-251).
+ * R_on_gpu / x_on_gpu: nonzero when the pointer is device memory. */
+int cos_buildr_get_r(cos_cf32* R, const cos_cf32* x, float d, int L, int Yavg, int M, int Nx,
+                     int Nz, int Nb, int R_on_gpu, int x_on_gpu);
+
+/* Replaces Solver::solve (RealTimeCapon/ISolver.h:16-29, RealTimeCapon/Solver.cpp:23-116):
+ * batch general complex systems A x = b, A column-major N x N (as the reference's kernels
+ * read it, RealTimeCapon/nvidia_solve.h:44-46), 1 <= N <= 72.  Gaussian elimination with
+ * partial pivoting.  *_on_gpu: nonzero when the pointer is device memory. */
+int cos_solver_solve(cos_cf32* x, const cos_cf32* A, const cos_cf32* b, int N, int batch,
+                     int x_on_gpu, int A_on_gpu, int b_on_gpu);
+
+/* Message for the last failure on this thread ("" if none). */
+const char* cos_capon_last_error(void);
+
+/* ----------------------------------------------------------------------------------
+ * Plan API (device pointers, caller's stream).  No reference counterpart: the
+ * reference keeps device buffers inside its Capon object (Capon.h:16-40); a plan is
+ * that object made explicit, one per thread of use.
+ * ---------------------------------------------------------------------------------- */
+
+typedef struct cos_capon_plan cos_capon_plan;
+
+enum {
+    COS_PREC_FP32 = 0,           /* FP32 covariance and solve */
+    COS_PREC_FP64 = 1,           /* plain per-pixel FP64: direct covariance, FP64 Cholesky */
+    COS_PREC_FP32_FALLBACK = 2   /* FP32, then FP64 for pixels flagged ill-conditioned */
+};
+
+typedef struct {
+    int Nx, Nz;          /* beams, range samples */
+    int Mstride;         /* complex elements per snapshot row in memory */
+    int M, L, off;       /* active elements, subarray length, first active element */
+    int Yavg;            /* temporal window 2*Yavg+1 */
+    float d;             /* relative diagonal loading */
+    const int* row_L;    /* optional device tables of length Nz indexed by the centre row */
+    const int* row_M;    /*   (override L, M, off, d per range row) */
+    const int* row_off;
+    const float* row_d;
+    int Lmax, Mmax;      /* bounds on the row tables (ignored without tables) */
+    int precision;       /* COS_PREC_* */
+    float cond_thresh;   /* fallback when 1 + L/d exceeds this (default 1e6 when <= 0) */
+    const float* steer_sin; /* optional device array of nsteer sin(theta) values */
+    int nsteer;
+} cos_capon_desc;
+
+int cos_capon_plan_create(cos_capon_plan** plan);
+int cos_capon_plan_destroy(cos_capon_plan* plan);
+
+/* One frame.  Outputs (device, any may be NULL): P [npx*S], z [npx*S], w [npx*S][wstride],
+ * R [npx][L*L] (uniform L only), flags [npx] (bit0 condition bound above threshold,
+ * bit1 non-positive pivot in FP32), with npx = Nx*(Nz-2*Yavg) and S = max(nsteer, 1).
+ * stream is a hipStream_t (NULL = default stream).  Asynchronous: returns after
+ * enqueueing; the FP64 fallback pass is enqueued on the same stream. */
+int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x,
+                  float* P, cos_cf32* z, cos_cf32* w, int wstride, cos_cf32* R,
+                  unsigned char* flags, void* stream);
+
+/* Number of pixels the last cos_capon_run sent to the FP64 fallback (synchronises the
+ * plan's stream). */
+int cos_capon_last_fallback_count(cos_capon_plan* plan, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COS_CAPON_H */
