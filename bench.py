@@ -35,6 +35,16 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
+def measured_traffic():
+    """HBM bytes per dispatch of the fused kernel from the committed PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", "capon_fused_f32_cfg2_traffic.txt")
+    try:
+        kv = dict(l.strip().split("=", 1) for l in open(path) if "=" in l and not l.startswith("#"))
+        return int(kv["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(x, budget_s=12.0):
     """The oracle's C FP64 restatement of the reference CPU path, all host cores, on a
     bounded sample of whole beams; rate scaled to pixels/s."""
@@ -139,7 +149,7 @@ def main():
                    "fallback_pixels": nfb},
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None, "kernel": "capon_kernel<float,16,2,5>",
+                     "traffic": measured_traffic(), "kernel": "capon_kernel<float,16,2,5>",
                      "kernel_ms": round(kern_ms, 4), "flop_per_pixel": FLOP_PER_PX},
         "accuracy": {"P_rel_err_max": float(err.max()), "P_rel_err_p999": float(np.quantile(err, 0.999)),
                      "sample": "256 beams x rows 900..915 vs FP64 oracle"},

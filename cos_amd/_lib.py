@@ -51,6 +51,14 @@ def lib() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m cos_amd.build`")
+    # PyTorch-ROCm ships its own libamdhip64.so.7.  Loading torch first makes the
+    # dynamic linker satisfy this library's libamdhip64.so.7 dependency with that same
+    # runtime, so device pointers and streams from torch are valid here and only one
+    # HIP runtime owns the GPU in the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, ip, fl = C.c_void_p, C.c_int, C.c_float
     L.cos_capon_get_capon.argtypes = [vp, vp, vp, vp, fl, ip, ip, ip, ip, ip, ip]

@@ -87,9 +87,10 @@ int check_args(float d, int L, int Yavg, int M, int Nx, int Nz, int Nb) {
   return 0;
 }
 
-cosk::CaponLaunch base_launch(const cos_capon_desc& ds) {
+cosk::CaponLaunch base_launch(const cos_capon_desc& ds, const void* x) {
   cosk::CaponLaunch a;
   std::memset(&a, 0, sizeof(a));
+  a.x = static_cast<const float2*>(x);
   a.Nx = ds.Nx;
   a.Nz = ds.Nz;
   a.Mstride = ds.Mstride;
@@ -103,6 +104,9 @@ cosk::CaponLaunch base_launch(const cos_capon_desc& ds) {
   a.rowOff = ds.row_off;
   a.rowD = ds.row_d;
   a.npix = ds.Nx * (ds.Nz - 2 * ds.Yavg);
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  a.Lcap = tables ? ds.Lmax : ds.L;
+  a.Mcap = tables ? ds.Mmax : ds.M;
   a.cond_thresh = ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6f;
   a.steer_sin = ds.steer_sin;
   a.nsteer = ds.nsteer;
@@ -165,7 +169,7 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   if (ds.nsteer < 0 || (ds.nsteer > 0 && !ds.steer_sin)) return fail(hipErrorInvalidValue, "bad steering");
   hipStream_t s = (hipStream_t)stream;
 
-  cosk::CaponLaunch a = base_launch(ds);
+  cosk::CaponLaunch a = base_launch(ds, x);
   a.P = P;
   a.z = (float2*)z;
   a.w = (float2*)w;
@@ -182,6 +186,8 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
     return 0;
   }
   const bool fb = ds.precision == COS_PREC_FP32_FALLBACK;
+  if (fb && ds.nsteer > 0) return fail(hipErrorInvalidValue, "FP64 fallback does not support steering");
+  if (fb && Lmax > 32) return fail(hipErrorInvalidValue, "FP64 fallback supports L <= 32");
   uint8_t* fl = (uint8_t*)flags;
   if (fb && !fl) {
     if ((e = plan->flags.ensure(a.npix)) != hipSuccess) return fail_hip(e, "flags buffer");
@@ -191,8 +197,6 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   e = cosk::launch_capon_f32(a, Lmax, Mmax, s, &why);
   if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
   if (!fb) return 0;
-  if (ds.nsteer > 0) return fail(hipErrorInvalidValue, "FP64 fallback does not support steering");
-  if (Lmax > 32) return fail(hipErrorInvalidValue, "FP64 fallback supports L <= 32");
   if ((e = plan->list.ensure(sizeof(int) * (size_t)a.npix)) != hipSuccess) return fail_hip(e, "list buffer");
   if ((e = plan->count.ensure(sizeof(int))) != hipSuccess) return fail_hip(e, "count buffer");
   e = cosk::compact_flags(fl, a.npix, plan->list.as<int>(), plan->count.as<int>(), s);
@@ -248,7 +252,7 @@ int cos_capon_get_capon_p(cos_cf32* z, cos_cf32* w, cos_cf32* R, float* P, const
     if (r) return r;
   } else {
     // Reference default beamspace: element-space loaded S_L, then B S_L B^H (Capon.cpp:285-402).
-    cosk::CaponLaunch a = base_launch(ds);
+    cosk::CaponLaunch a = base_launch(ds, pl->x.p);
     a.R = pl->R.as<float2>();
     a.Rstride = L * L;
     a.cov_only = 1;
@@ -314,8 +318,7 @@ int cos_buildr_get_r(cos_cf32* R, const cos_cf32* x, float d, int L, int Yavg, i
     cos_capon_desc ds;
     std::memset(&ds, 0, sizeof(ds));
     ds.Nx = Nx; ds.Nz = Nz; ds.Mstride = M; ds.M = M; ds.L = L; ds.Yavg = Yavg; ds.d = d;
-    cosk::CaponLaunch a = base_launch(ds);
-    a.x = (const float2*)xd;
+    cosk::CaponLaunch a = base_launch(ds, xd);
     a.R = (float2*)Rd;
     a.Rstride = L * L;
     a.cov_only = 1;

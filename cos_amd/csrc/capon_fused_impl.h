@@ -83,10 +83,20 @@ void capon_kernel(const CaponLaunch a, const int MP) {
   const int gsel = active ? gid : gid0;
   const int p = a.plist ? a.plist[gsel] : gsel;
   const int ix = p / Nzp, j = p - ix * Nzp, crow = j + Y;
-  const int L = a.rowL ? a.rowL[crow] : a.L;
-  const int M = a.rowM ? a.rowM[crow] : a.M;
-  const int off = a.rowOff ? a.rowOff[crow] : a.off;
-  const float dload = a.rowD ? a.rowD[crow] : a.d;
+  int L = a.rowL ? a.rowL[crow] : a.L;
+  int M = a.rowM ? a.rowM[crow] : a.M;
+  int off = a.rowOff ? a.rowOff[crow] : a.off;
+  float dload = a.rowD ? a.rowD[crow] : a.d;
+  // A row whose sizes break the launch's promises is computed as a harmless 1x1
+  // problem and its outputs are written as NaN, so no access can leave the buffers.
+  const bool valid = L >= 1 && L <= LMAX && L <= a.Lcap && M >= L && M <= a.Mcap && off >= 0 &&
+                     off + M <= a.Mstride && dload >= 0.f;
+  if (!valid) {
+    L = 1;
+    M = 1;
+    off = 0;
+    dload = 1.f;
+  }
   const int K = M - L + 1;
 
   // Stage the 2Y+1 snapshots of the window, zero padded to MP elements.
@@ -339,8 +349,9 @@ void capon_kernel(const CaponLaunch a, const int MP) {
 
     const size_t po = (size_t)p * nrhs + ia;
     if (active && g == 0) {
-      if (a.P) a.P[po] = (float)(T(1) / (norm * accP));
-      if (a.z) a.z[po] = make_float2((float)(accZ.x / accP), (float)(accZ.y / accP));
+      const float nanv = __builtin_nanf("");
+      if (a.P) a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
+      if (a.z) a.z[po] = valid ? make_float2((float)(accZ.x / accP), (float)(accZ.y / accP)) : make_float2(nanv, nanv);
     }
 
     if (a.w) {
@@ -393,6 +404,7 @@ void capon_kernel(const CaponLaunch a, const int MP) {
     uint8_t f = 0;
     if ((T(1) + (T)L / (T)dload) > (T)a.cond_thresh) f |= 1;
     if (notpd) f |= 2;
+    if (!valid) f = 4;
     a.flags[p] = f;
   }
   (void)pmin;
@@ -406,6 +418,7 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   const int TW = 2 * a.Y + 1;
   const size_t lds = (size_t)NG * TW * MP * sizeof(float2);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
+  if (!a.x) return hipErrorInvalidValue;
   const int blocks = (a.npix + NG - 1) / NG;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT>), dim3(blocks), dim3(64), lds, s, a, MP);

@@ -12,6 +12,7 @@ struct CaponLaunch {
   const float2* x;     // cube [Nx][Nz][Mstride] (complex float interleaved)
   int Nx, Nz, Mstride, Y;
   int L, M, off;       // uniform subarray length / active elements / first element
+  int Lcap, Mcap;      // bounds the host promised for every row (L <= Lcap, M <= Mcap)
   float d;             // uniform relative diagonal loading
   const int* rowL;     // optional per-centre-row tables, length Nz (override L/M/off/d)
   const int* rowM;
