@@ -57,6 +57,23 @@ template <> __device__ __forceinline__ double tiny_value<double>() { return 1.0e
 // T_: scalar type (float or double); G, CPL: lane group and columns per lane;
 // TM: compile-time bound on the window length 2Y+1 (0 = runtime, reload from LDS);
 // DIRECT: direct covariance sums instead of the displacement recursion.
+// (1/K) sum_k x_c[k+c] exp(-j pi k sin(theta)) for one column c of a steered pixel:
+// the subarray mean of the phase-rotated centre snapshot, expressed in the unrotated
+// element frame (read from global memory; only the steering path uses it).
+template <typename T>
+__device__ __noinline__ cplx<T> steered_mean(const float2* xc, int c, int K, int L, T sn, T invK) {
+  cplx<T> acc = {T(0), T(0)};
+  if (c < L) {
+    for (int k = 0; k < K; ++k) {
+      float s2, c2;
+      sincospif(-(float)(sn * (T)k), &s2, &c2);
+      const float2 xv = xc[k + c];
+      add_mul(acc, cplx<T>{(T)xv.x, (T)xv.y}, cplx<T>{(T)c2, (T)s2});
+    }
+  }
+  return {acc.x * invK, acc.y * invK};
+}
+
 // Occupancy target (waves per SIMD) the register allocator is held to.
 template <typename T, int G, int CPL> struct OccHint {
   static constexpr int value = sizeof(T) == 4 ? (G == 16 ? (CPL == 1 ? 4 : 2) : 1) : (CPL == 1 ? 2 : 1);
@@ -65,7 +82,7 @@ template <typename T, int G, int CPL> struct OccHint {
 template <typename T, int G, int CPL, int TM, bool DIRECT>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
                           amdgpu_waves_per_eu(OccHint<T, G, CPL>::value)))
-void capon_kernel(const CaponLaunch a, const int MP) {
+void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   using Gp = Grp<G>;
   constexpr int NG = Gp::NG, LMAX = G * CPL, NS = G * CPL * (CPL + 1) / 2;
   constexpr int TMR = TM > 0 ? TM : 1;
@@ -73,7 +90,7 @@ void capon_kernel(const CaponLaunch a, const int MP) {
 
   const int g = Gp::lane(), q = Gp::group();
   const int Y = a.Y, TW = 2 * Y + 1, Nzp = a.Nz - 2 * Y;
-  float2* xs = xs_all + q * TW * MP;
+  float2* xs = xs_all + q * GREG;
 
   const int npix = a.pcount ? min(*a.pcount, a.npix) : a.npix;
   const int gid0 = blockIdx.x * NG;
@@ -213,21 +230,22 @@ void capon_kernel(const CaponLaunch a, const int MP) {
   }
 
   // ---------------- trace, loading, padding ----------------
+  // tr(S) = sum_t sum_m |x_t[m]|^2 * #{(k, l): k + l = m, k < K, l < L}
   T trp = T(0);
-  static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
-    static_for<0, G * (S + 1)>([&](auto R_) {
-    constexpr int R = decltype(R_)::value;
-      trp += (cs[S] == R && R < L) ? A[slot<G, S, R>()].x : T(0);
-    });
-  });
+  for (int t = 0; t < TW; ++t) {
+    for (int m = g; m < M; m += G) {
+      const cplx<T> v = ldx<T>(xs + t * MP + m);
+      const int cnt = min(L - 1, m) - max(0, m - K + 1) + 1;
+      trp = fma((T)cnt, fma(v.x, v.x, v.y * v.y), trp);
+    }
+  }
   const T trS = Gp::sum(trp);
   const T delta = (T)dload * trS / (T)L;
   if (L < LMAX) {
     static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
+      constexpr int S = decltype(S_)::value;
       static_for<0, G * (S + 1)>([&](auto R_) {
-    constexpr int R = decltype(R_)::value;
+        constexpr int R = decltype(R_)::value;
         if (cs[S] >= L) A[slot<G, S, R>()] = {(cs[S] == R) ? T(1) : T(0), T(0)};
       });
     });
@@ -237,9 +255,9 @@ void capon_kernel(const CaponLaunch a, const int MP) {
   if (a.R && active) {
     float2* Rp = a.R + (size_t)p * a.Rstride;
     static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
+      constexpr int S = decltype(S_)::value;
       static_for<0, G * (S + 1)>([&](auto R_) {
-    constexpr int R = decltype(R_)::value;
+        constexpr int R = decltype(R_)::value;
         const int cc = cs[S];
         if (cc < L && R <= cc) {
           cplx<T> v = A[slot<G, S, R>()];
@@ -253,23 +271,35 @@ void capon_kernel(const CaponLaunch a, const int MP) {
 
   if (a.cov_only) return;
 
+  const T invK = T(1) / (T)K;
+  const bool steered = a.nsteer > 0;
+  // subarray mean of the centre snapshot, g_c = (1/K) sum_k x_c[k+c], for this lane's columns
+  cplx<T> gS[CPL];
+  static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+    cplx<T> acc = {T(0), T(0)};
+    if (!steered && cs[S] < L) {
+      const float2* xc = xs + Y * MP + cs[S];
+      for (int k = 0; k < K; ++k) {
+        const cplx<T> v = ldx<T>(xc + k);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+    }
+    gS[S] = {acc.x * invK, acc.y * invK};
+  });
+
   // ---------------- Cholesky U^H U = S_L ----------------
-  T pmin = big_value<T>(), pmax = T(0);
   bool notpd = false;
   static_for<0, LMAX>([&](auto Kk_) {
-    constexpr int Kk = decltype(Kk_)::value;
-    constexpr int k = Kk, sk = k / G, lk = k % G;
+    constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
     T piv = Gp::template bcast<lk>(A[slot<G, sk, k>()].x) + ((k < L) ? delta : T(0));
     const bool ok = piv > tiny_value<T>();
     notpd |= !ok;
     piv = ok ? piv : tiny_value<T>();
-    if (k < L) {
-      pmin = min(pmin, piv);
-      pmax = max(pmax, piv);
-    }
     const T inv = rsqrt_t<T>(piv);
     static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
+      constexpr int S = decltype(S_)::value;
       if constexpr (k < G * (S + 1)) {
         A[slot<G, S, k>()].x *= inv;
         A[slot<G, S, k>()].y *= inv;
@@ -277,126 +307,122 @@ void capon_kernel(const CaponLaunch a, const int MP) {
     });
     if (g == lk) A[slot<G, sk, k>()] = {piv * inv, inv};
     static_for<k + 1, LMAX>([&](auto Ii_) {
-    constexpr int Ii = decltype(Ii_)::value;
-      constexpr int i = Ii, si = i / G, li = i % G;
+      constexpr int i = decltype(Ii_)::value, si = i / G, li = i % G;
       const cplx<T> u = bcastc<G, li>(A[slot<G, si, k>()]);
       static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
+        constexpr int S = decltype(S_)::value;
         if constexpr (i < G * (S + 1)) sub_conj_mul(A[slot<G, S, i>()], u, A[slot<G, S, k>()]);
       });
     });
   });
 
+  // Strictly upper U[r][c] (r < c) packed at r + c(c+1)/2 into this group's LDS region,
+  // which held the snapshots (all reads of them are complete: the values above depend
+  // on them).  The back solve reads it row-wise.
+  const bool want_y = a.w || a.z;
+  cplx<T>* Upk = reinterpret_cast<cplx<T>*>(xs);
+  if (want_y) {
+    static_for<0, CPL>([&](auto S_) {
+      constexpr int S = decltype(S_)::value;
+      static_for<0, G * (S + 1)>([&](auto R_) {
+        constexpr int R = decltype(R_)::value;
+        const int cc = cs[S];
+        if (R < cc) Upk[R + cc * (cc + 1) / 2] = A[slot<G, S, R>()];
+      });
+    });
+  }
+
   const T norm = (T)K * (T)TW;
-  const int nrhs = a.nsteer > 0 ? a.nsteer : 1;
-  const float2* xc = xs + Y * MP;
-  const T invK = T(1) / (T)K;
+  const int nrhs = steered ? a.nsteer : 1;
+  const float2* xcg = a.x + ((size_t)ix * a.Nz + (j + Y)) * a.Mstride + off;   // centre row (steering)
 
   for (int ia = 0; ia < nrhs; ++ia) {
-    const T sn = a.nsteer > 0 ? (T)a.steer_sin[ia] : T(0);
-    // right-hand sides a (ones or steering vector) and g (subarray mean of the centre row)
-    cplx<T> b1[CPL], b2[CPL], zA[CPL];
+    const T sn = steered ? (T)a.steer_sin[ia] : T(0);
+    cplx<T> b1[CPL], zA[CPL], gcur[CPL];
     static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
+      constexpr int S = decltype(S_)::value;
       const int cc = cs[S];
       zA[S] = {T(0), T(0)};
-      cplx<T> gs = {T(0), T(0)};
-      if (a.nsteer > 0) {
+      b1[S] = {T(1), T(0)};
+      gcur[S] = gS[S];
+      if (steered) {
         float sv, cv;
         sincospif((float)(sn * (T)cc), &sv, &cv);
         b1[S] = {(T)cv, (T)sv};
-        for (int k = 0; k < K; ++k) {
-          float s2, c2;
-          sincospif(-(float)(sn * (T)k), &s2, &c2);
-          add_mul(gs, ldx<T>(xc + k + cc), cplx<T>{(T)c2, (T)s2});
-        }
-      } else {
-        b1[S] = {T(1), T(0)};
-        for (int k = 0; k < K; ++k) {
-          const cplx<T> v = ldx<T>(xc + k + cc);
-          gs.x += v.x;
-          gs.y += v.y;
-        }
+        gcur[S] = steered_mean<T>(xcg, cc, K, L, sn, invK);
       }
-      if (cc >= L) {
-        b1[S] = {T(0), T(0)};
-        gs = {T(0), T(0)};
-      }
-      b2[S] = {gs.x * invK, gs.y * invK};
+      if (cc >= L) b1[S] = {T(0), T(0)};
     });
 
+    // forward solve U^H v = a; |v|^2 = a^H S_L^-1 a
     T accP = T(0);
-    cplx<T> accZ = {T(0), T(0)};
     static_for<0, LMAX>([&](auto Kk_) {
-    constexpr int Kk = decltype(Kk_)::value;
-      constexpr int k = Kk, sk = k / G, lk = k % G;
+      constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
       const T inv = Gp::template bcast<lk>(A[slot<G, sk, k>()].y);
       cplx<T> v1 = bcastc<G, lk>(b1[sk]);
-      cplx<T> v2 = bcastc<G, lk>(b2[sk]);
-      v1.x *= inv; v1.y *= inv;
-      v2.x *= inv; v2.y *= inv;
+      v1.x *= inv;
+      v1.y *= inv;
       accP = fma(v1.x, v1.x, fma(v1.y, v1.y, accP));
-      add_conj_mul(accZ, v1, v2);
       if (g == lk) zA[sk] = v1;
       static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
-        if constexpr (k < G * (S + 1)) {
-          sub_conj_mul(b1[S], A[slot<G, S, k>()], v1);
-          sub_conj_mul(b2[S], A[slot<G, S, k>()], v2);
-        }
+        constexpr int S = decltype(S_)::value;
+        if constexpr (k < G * (S + 1)) sub_conj_mul(b1[S], A[slot<G, S, k>()], v1);
       });
     });
 
     const size_t po = (size_t)p * nrhs + ia;
-    if (active && g == 0) {
-      const float nanv = __builtin_nanf("");
-      if (a.P) a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
-      if (a.z) a.z[po] = valid ? make_float2((float)(accZ.x / accP), (float)(accZ.y / accP)) : make_float2(nanv, nanv);
-    }
+    const float nanv = __builtin_nanf("");
+    if (active && g == 0 && a.P) a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
 
-    if (a.w) {
-      // back solve U y = U^-H a, one row per step, reduction over the group
-      cplx<T> y[CPL];
+    if (want_y) {
+      // back solve U y = v with the rows of U read from LDS; column-oriented:
+      // y_j = c_j / U_jj, then c_i -= U[i][j] y_j for the rows i < j this lane owns.
+      cplx<T> y[CPL], cr[CPL];
       static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value; y[S] = {T(0), T(0)}; });
-      static_for<LMAX - 1, -1, -1>([&](auto Cc_) {
-    constexpr int Cc = decltype(Cc_)::value;
-        constexpr int c = Cc, sc = c / G, lc = c % G;
-        cplx<T> part = {T(0), T(0)};
-        static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
-          if constexpr (c < G * (S + 1)) {
-            cplx<T> pr = {T(0), T(0)};
-            add_mul(pr, A[slot<G, S, c>()], y[S]);
-            const bool use = cs[S] > c;
-            part.x += use ? pr.x : T(0);
-            part.y += use ? pr.y : T(0);
-          }
-        });
-        part.x = Gp::sum(part.x);
-        part.y = Gp::sum(part.y);
-        const T inv = A[slot<G, sc, c>()].y;
-        const cplx<T> val = {(zA[sc].x - part.x) * inv, (zA[sc].y - part.y) * inv};
-        if (g == lc) y[sc] = val;
+        constexpr int S = decltype(S_)::value;
+        y[S] = {T(0), T(0)};
+        cr[S] = zA[S];
       });
-      if (active) {
-        const T sc = T(1) / accP;
-        static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
-          const int cc = cs[S];
-          if (cc < L) {
-            cplx<T> wv = {y[S].x * sc, y[S].y * sc};
-            if (a.nsteer > 0) {
-              float sv, cv;
-              sincospif(-(float)(sn * (T)cc), &sv, &cv);
-              cplx<T> r = {T(0), T(0)};
-              add_mul(r, wv, cplx<T>{(T)cv, (T)sv});
-              wv = r;
+      static_for<LMAX - 1, -1, -1>([&](auto Cc_) {
+        constexpr int jj = decltype(Cc_)::value, sj = jj / G, lj = jj % G;
+        const T inv = A[slot<G, sj, jj>()].y;                 // owner lane: 1/U_jj
+        const cplx<T> val = {cr[sj].x * inv, cr[sj].y * inv};
+        if (g == lj) y[sj] = val;
+        if constexpr (jj > 0) {
+          const cplx<T> yb = bcastc<G, lj>(val);
+          static_for<0, CPL>([&](auto S_) {
+            constexpr int S = decltype(S_)::value;
+            if constexpr (G * S < jj) {                      // some row i = g + G*S < jj
+              const int ii = min(cs[S], jj);
+              const cplx<T> u = Upk[ii + jj * (jj + 1) / 2];
+              cplx<T> t = cr[S];
+              add_mul(t, cplx<T>{-u.x, -u.y}, yb);
+              if (cs[S] < jj) cr[S] = t;
             }
-            a.w[po * a.wstride + cc] = make_float2((float)wv.x, (float)wv.y);
-          }
-        });
-      }
+          });
+        }
+      });
+      // w = y / (a^H y), a^H y = |v|^2;  z = w^H g
+      const T sc = T(1) / accP;
+      cplx<T> zp = {T(0), T(0)};
+      static_for<0, CPL>([&](auto S_) {
+        constexpr int S = decltype(S_)::value;
+        const int cc = cs[S];
+        cplx<T> wv = {y[S].x * sc, y[S].y * sc};
+        if (cc < L) add_conj_mul(zp, wv, gcur[S]);
+        if (steered) {
+          float sv, cv;
+          sincospif(-(float)(sn * (T)cc), &sv, &cv);
+          cplx<T> r = {T(0), T(0)};
+          add_mul(r, wv, cplx<T>{(T)cv, (T)sv});
+          wv = r;
+        }
+        if (a.w && active && cc < L) a.w[po * a.wstride + cc] = make_float2((float)wv.x, (float)wv.y);
+      });
+      zp.x = Gp::sum(zp.x);
+      zp.y = Gp::sum(zp.y);
+      if (active && g == 0 && a.z)
+        a.z[po] = valid ? make_float2((float)zp.x, (float)zp.y) : make_float2(nanv, nanv);
     }
   }
 
@@ -407,8 +433,6 @@ void capon_kernel(const CaponLaunch a, const int MP) {
     if (!valid) f = 4;
     a.flags[p] = f;
   }
-  (void)pmin;
-  (void)pmax;
 }
 
 template <typename T, int G, int CPL, int TM, bool DIRECT>
@@ -416,12 +440,15 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   constexpr int NG = 64 / G, LMAX = G * CPL;
   const int MP = ((Mmax + LMAX + 3) / 4) * 4;
   const int TW = 2 * a.Y + 1;
-  const size_t lds = (size_t)NG * TW * MP * sizeof(float2);
+  // per-group LDS region: the window's snapshots, later the packed upper U
+  const int upk = LMAX * (LMAX + 1) / 2 * (int)(sizeof(cplx<T>) / sizeof(float2));
+  const int GREG = TW * MP > upk ? TW * MP : upk;
+  const size_t lds = (size_t)NG * GREG * sizeof(float2);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   if (!a.x) return hipErrorInvalidValue;
   const int blocks = (a.npix + NG - 1) / NG;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT>), dim3(blocks), dim3(64), lds, s, a, MP);
+  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT>), dim3(blocks), dim3(64), lds, s, a, MP, GREG);
   return hipGetLastError();
 }
 

@@ -29,15 +29,17 @@ template <typename T> struct cplx { T x, y; };
 template <typename T> __device__ __forceinline__ cplx<T> mk(T a, T b) { return {a, b}; }
 
 // ---- 32-bit DPP / lane primitives ---------------------------------------------------
-template <int CTRL, bool BOUND = false>
+// mov_dpp leaves lanes whose source is outside the row at 0 (bound_ctrl) and needs
+// no zero-initialised destination, unlike update_dpp with an explicit old value.
+template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, BOUND));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
-template <int CTRL, bool BOUND = false>
+template <int CTRL>
 __device__ __forceinline__ double dpp(double v) {
   int2 p = *reinterpret_cast<int2*>(&v);
-  p.x = __builtin_amdgcn_update_dpp(0, p.x, CTRL, 0xF, 0xF, BOUND);
-  p.y = __builtin_amdgcn_update_dpp(0, p.y, CTRL, 0xF, 0xF, BOUND);
+  p.x = __builtin_amdgcn_mov_dpp(p.x, CTRL, 0xF, 0xF, true);
+  p.y = __builtin_amdgcn_mov_dpp(p.y, CTRL, 0xF, 0xF, true);
   return *reinterpret_cast<double*>(&p);
 }
 __device__ __forceinline__ float rdlane(float v, int l) {
@@ -67,10 +69,10 @@ template <> struct Grp<16> {
   template <typename T> __device__ static T ror1(T v) { return dpp<DPP_ROW_ROR + 1>(v); }
   // Sum over the 16 lanes of the row, returned in every lane.
   template <typename T> __device__ static T sum(T v) {
-    v += dpp<DPP_ROW_SHR + 1, true>(v);
-    v += dpp<DPP_ROW_SHR + 2, true>(v);
-    v += dpp<DPP_ROW_SHR + 4, true>(v);
-    v += dpp<DPP_ROW_SHR + 8, true>(v);
+    v += dpp<DPP_ROW_SHR + 1>(v);
+    v += dpp<DPP_ROW_SHR + 2>(v);
+    v += dpp<DPP_ROW_SHR + 4>(v);
+    v += dpp<DPP_ROW_SHR + 8>(v);
     return dpp<DPP_ROW_NEWBCAST + 15>(v);
   }
 };
@@ -83,10 +85,10 @@ template <> struct Grp<64> {
   template <int S, typename T> __device__ static T bcast(T v) { return rdlane(v, S); }
   template <typename T> __device__ static T ror1(T v) { return dpp<DPP_WAVE_ROR1>(v); }
   template <typename T> __device__ static T sum(T v) {
-    v += dpp<DPP_ROW_SHR + 1, true>(v);
-    v += dpp<DPP_ROW_SHR + 2, true>(v);
-    v += dpp<DPP_ROW_SHR + 4, true>(v);
-    v += dpp<DPP_ROW_SHR + 8, true>(v);
+    v += dpp<DPP_ROW_SHR + 1>(v);
+    v += dpp<DPP_ROW_SHR + 2>(v);
+    v += dpp<DPP_ROW_SHR + 4>(v);
+    v += dpp<DPP_ROW_SHR + 8>(v);
     return rdlane(v, 15) + rdlane(v, 31) + rdlane(v, 47) + rdlane(v, 63);
   }
 };
@@ -138,7 +140,8 @@ __device__ __forceinline__ void add_conj_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
 }
 
 template <typename T> __device__ __forceinline__ T rsqrt_t(T v);
-template <> __device__ __forceinline__ float rsqrt_t<float>(float v) { return 1.0f / __fsqrt_rn(v); }
+// v_rsq_f32 (1 ulp); the FP32 path's accuracy budget is set by the covariance sums.
+template <> __device__ __forceinline__ float rsqrt_t<float>(float v) { return __builtin_amdgcn_rsqf(v); }
 template <> __device__ __forceinline__ double rsqrt_t<double>(double v) { return 1.0 / sqrt(v); }
 
 }  // namespace cosk
