@@ -80,9 +80,8 @@ template <typename T, int G, int CPL> struct OccHint {
 };
 
 template <typename T, int G, int CPL, int TM, bool DIRECT>
-__global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
-                          amdgpu_waves_per_eu(OccHint<T, G, CPL>::value)))
-void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
+__device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, const int GREG,
+                                            const int gid0, const int npix) {
   using Gp = Grp<G>;
   constexpr int NG = Gp::NG, LMAX = G * CPL, NS = G * CPL * (CPL + 1) / 2;
   constexpr int TMR = TM > 0 ? TM : 1;
@@ -90,11 +89,6 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
 
   const int g = Gp::lane(), q = Gp::group();
   const int Y = a.Y, TW = 2 * Y + 1, Nzp = a.Nz - 2 * Y;
-  float2* xs = xs_all + q * GREG;
-
-  const int npix = a.pcount ? min(*a.pcount, a.npix) : a.npix;
-  const int gid0 = blockIdx.x * NG;
-  if (gid0 >= npix) return;  // whole wave idle (uniform)
   const int gid = gid0 + q;
   const bool active = gid < npix;
   const int gsel = active ? gid : gid0;
@@ -116,10 +110,33 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   }
   const int K = M - L + 1;
 
-  // Stage the 2Y+1 snapshots of the window, zero padded to MP elements.
-  for (int t = 0; t < TW; ++t) {
-    const float2* src = a.x + ((size_t)ix * a.Nz + (j + t)) * a.Mstride + off;
-    for (int m = g; m < MP; m += G) xs[t * MP + m] = (m < M) ? src[m] : make_float2(0.f, 0.f);
+  // Shared rows: when the wave's NG pixels are consecutive in one beam with the same
+  // aperture (every wave of a uniform frame whose row count is a multiple of NG), their
+  // windows overlap in NG + 2Y rows, which are staged once; group q then uses rows
+  // q .. q+2Y.  Otherwise each group stages its own 2Y+1 rows.
+  const int NR = NG + TW - 1;
+  bool shared = false;
+  if constexpr (NG > 1) {
+    const int p0 = __builtin_amdgcn_readfirstlane(p);
+    const int L0 = __builtin_amdgcn_readfirstlane(L), M0 = __builtin_amdgcn_readfirstlane(M);
+    const int o0 = __builtin_amdgcn_readfirstlane(off);
+    const int j0 = p0 - (p0 / Nzp) * Nzp;
+    const bool mine = active && p == p0 + q && L == L0 && M == M0 && off == o0 && j0 + NG <= Nzp;
+    shared = __all(mine) && (NR * MP + NR * LMAX * (int)(sizeof(cplx<T>) / sizeof(float2)) <= NG * GREG);
+  }
+  float2* xs = shared ? xs_all + q * MP : xs_all + q * GREG;
+  if (shared) {
+    const float2* src0 = a.x + ((size_t)(p - q) / Nzp * a.Nz + (j - q)) * a.Mstride + off;
+    for (int t = 0; t < NR; ++t) {
+      const float2* src = src0 + (size_t)t * a.Mstride;
+      for (int m = threadIdx.x; m < MP; m += 64) xs_all[t * MP + m] = (m < M) ? src[m] : make_float2(0.f, 0.f);
+    }
+  } else {
+    // Stage the 2Y+1 snapshots of the window, zero padded to MP elements.
+    for (int t = 0; t < TW; ++t) {
+      const float2* src = a.x + ((size_t)ix * a.Nz + (j + t)) * a.Mstride + off;
+      for (int m = g; m < MP; m += G) xs[t * MP + m] = (m < M) ? src[m] : make_float2(0.f, 0.f);
+    }
   }
   __syncthreads();
 
@@ -152,13 +169,60 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   } else {
     cplx<T> s0[CPL];
     static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value; s0[S] = {T(0), T(0)}; });
-    for (int t = 0; t < TW; ++t) {
-      const float2* xr = xs + t * MP;
-      for (int k = 0; k < K; ++k) {
-        const cplx<T> u = ldx<T>(xr + k);
+      constexpr int S = decltype(S_)::value;
+      s0[S] = {T(0), T(0)};
+    });
+    if (shared) {
+      // Per-row autocorrelations rho_t(c) = sum_{k<K} x_t[k] conj(x_t[k+c]) for the NR
+      // shared rows, each computed once by the wave (lane task: one row, four lags, a
+      // sliding window of four values), then S[0][c] = sum of the group's 2Y+1 rows.
+      cplx<T>* rho = reinterpret_cast<cplx<T>*>(xs_all + NR * MP);
+      constexpr int NLB = LMAX / 4;
+      for (int task = threadIdx.x; task < NR * NLB; task += 64) {
+        const int t = task / NLB, c0 = 4 * (task - t * NLB);
+        const float2* xr = xs_all + t * MP;
+        cplx<T> acc[4], wv[4];
+        static_for<0, 4>([&](auto U_) {
+          constexpr int U = decltype(U_)::value;
+          acc[U] = {T(0), T(0)};
+          wv[U] = ldx<T>(xr + c0 + U);
+        });
+        // K rounded up to a multiple of 4; the extra terms read zero padding (a = 0)
+        for (int k = 0; k < K; k += 4) {
+          static_for<0, 4>([&](auto V_) {
+            constexpr int V = decltype(V_)::value;
+            const cplx<T> u = (k + V < K) ? ldx<T>(xr + k + V) : cplx<T>{T(0), T(0)};
+            static_for<0, 4>([&](auto U_) {
+              constexpr int U = decltype(U_)::value;
+              add_mul_conj(acc[U], u, wv[(U + V) % 4]);
+            });
+            wv[V] = ldx<T>(xr + k + V + c0 + 4);
+          });
+        }
+        static_for<0, 4>([&](auto U_) {
+          constexpr int U = decltype(U_)::value;
+          rho[t * LMAX + c0 + U] = acc[U];
+        });
+      }
+      __syncthreads();
+      for (int t = 0; t < TW; ++t) {
         static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value; add_mul_conj(s0[S], u, ldx<T>(xr + k + cs[S])); });
+          constexpr int S = decltype(S_)::value;
+          const cplx<T> r = rho[(q + t) * LMAX + cs[S]];
+          s0[S].x += r.x;
+          s0[S].y += r.y;
+        });
+      }
+    } else {
+      for (int t = 0; t < TW; ++t) {
+        const float2* xr = xs + t * MP;
+        for (int k = 0; k < K; ++k) {
+          const cplx<T> u = ldx<T>(xr + k);
+          static_for<0, CPL>([&](auto S_) {
+            constexpr int S = decltype(S_)::value;
+            add_mul_conj(s0[S], u, ldx<T>(xr + k + cs[S]));
+          });
+        }
       }
     }
     static_for<0, CPL>([&](auto S_) {
@@ -320,7 +384,7 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   // which held the snapshots (all reads of them are complete: the values above depend
   // on them).  The back solve reads it row-wise.
   const bool want_y = a.w || a.z;
-  cplx<T>* Upk = reinterpret_cast<cplx<T>*>(xs);
+  cplx<T>* Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
   if (want_y) {
     static_for<0, CPL>([&](auto S_) {
       constexpr int S = decltype(S_)::value;
@@ -435,10 +499,30 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   }
 }
 
-template <typename T, int G, int CPL, int TM, bool DIRECT>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
+                          amdgpu_waves_per_eu(OccHint<T, G, CPL>::value)))
+void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
+  // PERSIST: grid-stride over groups of NG pixels, so a launch over a device-side pixel
+  // list (the FP64 fallback) can use a bounded grid; otherwise one block per group (a
+  // loop costs the full-frame kernels registers).
+  constexpr int NGk = Grp<G>::NG;
+  const int npix = a.pcount ? min(*a.pcount, a.npix) : a.npix;
+  if constexpr (PERSIST) {
+    for (int gid0 = blockIdx.x * NGk; gid0 < npix; gid0 += gridDim.x * NGk)
+      capon_group<T, G, CPL, TM, DIRECT>(a, MP, GREG, gid0, npix);
+  } else {
+    // one group of pixels per block (a loop here costs the FP32 kernel its registers)
+    const int gid0 = blockIdx.x * NGk;
+    if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT>(a, MP, GREG, gid0, npix);
+  }
+}
+
+
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   constexpr int NG = 64 / G, LMAX = G * CPL;
-  const int MP = ((Mmax + LMAX + 3) / 4) * 4;
+  const int MP = ((Mmax + LMAX + 4 + 3) / 4) * 4;   // zero padding covers every lag read
   const int TW = 2 * a.Y + 1;
   // per-group LDS region: the window's snapshots, later the packed upper U
   const int upk = LMAX * (LMAX + 1) / 2 * (int)(sizeof(cplx<T>) / sizeof(float2));
@@ -446,9 +530,10 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   const size_t lds = (size_t)NG * GREG * sizeof(float2);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   if (!a.x) return hipErrorInvalidValue;
-  const int blocks = (a.npix + NG - 1) / NG;
+  int blocks = (a.npix + NG - 1) / NG;
+  if (PERSIST && blocks > 2048) blocks = 2048;   // list length known only on the device
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT>), dim3(blocks), dim3(64), lds, s, a, MP, GREG);
+  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST>), dim3(blocks), dim3(64), lds, s, a, MP, GREG);
   return hipGetLastError();
 }
 

@@ -1,0 +1,8 @@
+// Explicit instantiations of the FP64 per-pixel kernel (direct covariance), 32 columns:
+// full-frame launch and the bounded-grid launch over the fallback pixel list.
+#include "capon_fused_impl.h"
+
+namespace cosk {
+template hipError_t launch_one<double, 16, 2, 0, true, false>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<double, 16, 2, 0, true, true>(const CaponLaunch&, int, hipStream_t);
+}  // namespace cosk
