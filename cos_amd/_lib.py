@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libcos_capon.so")
 EXPORTS = (
     "cos_capon_get_capon", "cos_capon_get_capon_p", "cos_buildr_get_r", "cos_solver_solve",
     "cos_capon_last_error", "cos_capon_plan_create", "cos_capon_plan_destroy", "cos_capon_run",
-    "cos_capon_last_fallback_count",
+    "cos_capon_last_fallback_count", "cos_capon_beamspace_run",
 )
 
 COS_PREC_FP32 = 0
@@ -31,7 +31,7 @@ class CaponDesc(C.Structure):
         ("row_L", C.c_void_p), ("row_M", C.c_void_p), ("row_off", C.c_void_p), ("row_d", C.c_void_p),
         ("Lmax", C.c_int), ("Mmax", C.c_int),
         ("precision", C.c_int), ("cond_thresh", C.c_float),
-        ("steer_sin", C.c_void_p), ("nsteer", C.c_int),
+        ("steer_sin", C.c_void_p), ("nsteer", C.c_int), ("graph", C.c_int),
     ]
 
 
@@ -71,6 +71,7 @@ def lib() -> C.CDLL:
     L.cos_capon_plan_destroy.argtypes = [vp]
     L.cos_capon_run.argtypes = [vp, C.POINTER(CaponDesc), vp, vp, vp, vp, ip, vp, vp, vp]
     L.cos_capon_last_fallback_count.argtypes = [vp, C.POINTER(C.c_int)]
+    L.cos_capon_beamspace_run.argtypes = [vp, C.POINTER(CaponDesc), ip, vp, vp, vp, vp, vp, ip, vp]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.cos_capon_last_error.restype = C.c_char_p

@@ -141,13 +141,15 @@ class CaponPlan:
             self._h = None
 
     def run(self, x, d, L, Yavg, *, M=None, off=0, precision="fp32", want=("P", "z", "w"),
-            rows=None, steer_sin=None, cond_thresh=1e6, out=None, stream=None):
+            rows=None, steer_sin=None, cond_thresh=1e6, out=None, stream=None, graph=False):
         """Run one frame.
 
         x: complex64 CUDA tensor [Nx][Nz][Mstride].  rows: optional dict of CUDA tensors
         L, M, off (int32) and d (float32), length Nz, indexed by the centre row.
         precision: "fp32", "fp64" or "fp32+fallback".  steer_sin: optional float32 CUDA
-        tensor of sin(theta) values.  Returns a dict of output tensors.
+        tensor of sin(theta) values.  graph: capture the frame's launches in a hipGraph
+        on the first call and replay it while the descriptor and all tensors are the same
+        (pass the same ``out`` dict to keep the pointers).  Returns a dict of output tensors.
         """
         import torch
 
@@ -161,6 +163,7 @@ class CaponPlan:
         ds.M = int(M if M is not None else Mstride)
         ds.L, ds.off, ds.d = int(L), int(off), float(d)
         ds.precision, ds.cond_thresh = prec, float(cond_thresh)
+        ds.graph = int(bool(graph))
         Lmax = int(L)
         keep = []
         if rows is not None:
@@ -195,6 +198,35 @@ class CaponPlan:
             stream = torch.cuda.current_stream(dev).cuda_stream
         check(lib().cos_capon_run(self._h, C.byref(ds), x.data_ptr(), ptr("P"), ptr("z"), ptr("w"),
                                   wstride, ptr("R"), ptr("flags"), stream))
+        return o
+
+    def run_beamspace(self, x, d, L, Yavg, Nb, *, M=None, off=0, want=("P", "z"), out=None, stream=None):
+        """Beamspace Capon with the sliding DFT over Nb Butler beams (loading in beamspace).
+        x: complex64 CUDA tensor [Nx][Nz][Mstride].  want: any of P, z, w_bs, w (element
+        space).  Returns a dict of output tensors."""
+        import torch
+
+        if not (x.is_cuda and x.dtype == torch.complex64 and x.is_contiguous()):
+            raise ValueError("x must be a contiguous complex64 CUDA tensor")
+        Nx, Nz, Mstride = x.shape
+        ds = CaponDesc()
+        ds.Nx, ds.Nz, ds.Mstride, ds.Yavg = Nx, Nz, Mstride, int(Yavg)
+        ds.M = int(M if M is not None else Mstride)
+        ds.L, ds.off, ds.d = int(L), int(off), float(d)
+        npx = Nx * (Nz - 2 * Yavg)
+        dev = x.device
+        o = dict(out or {})
+        shapes = {"P": ((npx,), torch.float32), "z": ((npx,), torch.complex64),
+                  "w_bs": ((npx, Nb), torch.complex64), "w": ((npx, L), torch.complex64)}
+        for k in want:
+            if k not in o:
+                o[k] = torch.empty(shapes[k][0], dtype=shapes[k][1], device=dev)
+        ptr = lambda k: o[k].data_ptr() if k in o else None
+        wstride = o["w"].shape[-1] if "w" in o else 0
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        check(lib().cos_capon_beamspace_run(self._h, C.byref(ds), int(Nb), x.data_ptr(), ptr("P"), ptr("z"),
+                                            ptr("w_bs"), ptr("w"), wstride, stream))
         return o
 
     def fallback_count(self) -> int:

@@ -1,0 +1,242 @@
+// Beamspace Capon with a sliding DFT (sonar workloads: many beams, small Nb).
+//
+// Semantics (paths relative to the reference root):
+//   beams      RealTimeCapon/capon_kernel.cu:47-74 -- beta(b) = b for b <= Nb/2, else
+//              b + L - 2(Nb/2) - 1; B[b][l] = L^-1/2 exp(-j 2 pi beta(b) l / L)
+//   transform  RealTimeCapon/sliding_beamspace_kernel.cu:26-81 -- v_{k,b} = sum_l B[b][l] x[k+l]
+//              by S_{k+1} = (S_k + x[k+L] - x[k]) exp(+j 2 pi beta / L)
+//   covariance R_bs = (1/(K(2Y+1))) sum_t sum_k v v^H, loading d tr(R_bs)/Nb
+//              (framework/beamformer/capon/getCaponAllC.pyx:367-385, loading in beamspace)
+//   solve      a_bs = B 1 = sqrt(L) e_0; w_bs = R_bs^-1 a_bs / (a_bs^H R_bs^-1 a_bs)
+//   outputs    P = 1/(a_bs^H R_bs^-1 a_bs); z = w_bs^H s_c with s_c = (1/K) sum_k v_{c,k}
+//              the transformed subarray mean of the centre snapshot
+//              (RealTimeCapon/capon_kernel.cu:162-233); element-space w = B^H w_bs
+//              (RealTimeCapon/Capon.cpp:415).
+//
+// Work split.  A block owns one beam and a run of range rows.  Phase 1: one thread per
+// snapshot row runs the sliding DFT across its K subarrays and accumulates that row's
+// Nb x Nb Gram and its subarray mean; both go to LDS, so each snapshot is transformed once
+// and shared by the 2Y+1 pixels whose window contains it.  Phase 2: one thread per pixel
+// sums its window's Grams, factors the Nb x Nb system in registers and writes the outputs.
+// The cube is read once: the kernel is bounded by HBM and FP32 VALU together.
+#include "cos_device.h"
+#include "beamspace.h"
+
+namespace cosk {
+
+template <int NB>
+struct BsTri { static constexpr int N = NB * (NB + 1) / 2; };
+
+template <int NB, int RB>
+__global__ __launch_bounds__(RB) void beamspace_capon_kernel(const BeamspaceLaunch a) {
+  constexpr int NT = BsTri<NB>::N;
+  __shared__ float2 gram[RB][NT];
+  __shared__ float2 smean[RB][NB];
+  const int Y = a.Y, TW = 2 * Y + 1, Nzp = a.Nz - 2 * Y, TP = RB - 2 * Y;
+  const int ix = blockIdx.y;
+  const int t0 = blockIdx.x * TP;
+  const int r = threadIdx.x;
+  const int L = a.L, K = a.M - a.L + 1;
+  const float rsL = a.rsqrtL;
+
+  // ---------------- phase 1: sliding DFT and per-snapshot Gram ----------------
+  {
+    const int t = t0 + r;
+    float2 G[NT];
+    float2 sm[NB];
+#pragma unroll
+    for (int e = 0; e < NT; ++e) G[e] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) sm[b] = make_float2(0.f, 0.f);
+    if (t < a.Nz) {
+      const float2* x = a.x + ((size_t)ix * a.Nz + t) * a.Mstride + a.off;
+      float2 S[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) S[b] = make_float2(0.f, 0.f);
+      for (int l = 0; l < L; ++l) {
+        const float2 xv = x[l];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const float2 tw = a.tw[b * L + l];
+          S[b].x = fmaf(xv.x, tw.x, fmaf(-xv.y, tw.y, S[b].x));
+          S[b].y = fmaf(xv.x, tw.y, fmaf(xv.y, tw.x, S[b].y));
+        }
+      }
+      for (int k = 0; k < K; ++k) {
+        if (k > 0) {
+          const float2 xin = x[k + L - 1], xout = x[k - 1];
+          const float2 dlt = make_float2(xin.x - xout.x, xin.y - xout.y);
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const float2 u = make_float2(S[b].x + dlt.x, S[b].y + dlt.y);
+            const float2 om = a.omega[b];
+            S[b] = make_float2(fmaf(u.x, om.x, -u.y * om.y), fmaf(u.x, om.y, u.y * om.x));
+          }
+        }
+        float2 v[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          v[b] = make_float2(S[b].x * rsL, S[b].y * rsL);
+          sm[b].x += v[b].x;
+          sm[b].y += v[b].y;
+        }
+        int e = 0;
+#pragma unroll
+        for (int b1 = 0; b1 < NB; ++b1) {
+#pragma unroll
+          for (int b2 = b1; b2 < NB; ++b2, ++e) {   // G[b1][b2] += v[b1] conj(v[b2])
+            G[e].x = fmaf(v[b1].x, v[b2].x, fmaf(v[b1].y, v[b2].y, G[e].x));
+            G[e].y = fmaf(v[b1].y, v[b2].x, fmaf(-v[b1].x, v[b2].y, G[e].y));
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < NT; ++e) gram[r][e] = G[e];
+    const float invK = 1.f / (float)K;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) smean[r][b] = make_float2(sm[b].x * invK, sm[b].y * invK);
+  }
+  __syncthreads();
+
+  // ---------------- phase 2: one pixel per thread ----------------
+  const int j = t0 + r;
+  if (r >= TP || j >= Nzp) return;
+  float2 R[NT];
+#pragma unroll
+  for (int e = 0; e < NT; ++e) R[e] = make_float2(0.f, 0.f);
+  for (int u = 0; u < TW; ++u) {
+#pragma unroll
+    for (int e = 0; e < NT; ++e) {
+      const float2 gv = gram[r + u][e];
+      R[e].x += gv.x;
+      R[e].y += gv.y;
+    }
+  }
+  const float nrm = 1.f / ((float)K * (float)TW);
+  float tr = 0.f;
+  {
+    int e = 0;
+#pragma unroll
+    for (int b1 = 0; b1 < NB; ++b1)
+#pragma unroll
+      for (int b2 = b1; b2 < NB; ++b2, ++e) {
+        R[e].x *= nrm;
+        R[e].y *= nrm;
+        if (b1 == b2) tr += R[e].x;
+      }
+  }
+  const float dl = a.d * tr / (float)NB;
+  // Cholesky U^H U = R_bs + dl I, upper triangle in place (row-major packed)
+  float invd[NB];
+  {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int ii = i * NB - i * (i - 1) / 2;         // packed index of (i, i)
+      float piv = R[ii].x + dl;
+      piv = piv > 1e-30f ? piv : 1e-30f;
+      const float inv = __builtin_amdgcn_rsqf(piv);
+      invd[i] = inv;
+#pragma unroll
+      for (int c = i + 1; c < NB; ++c) {
+        R[ii + c - i].x *= inv;
+        R[ii + c - i].y *= inv;
+      }
+#pragma unroll
+      for (int i2 = i + 1; i2 < NB; ++i2) {
+        const float2 ui = R[ii + i2 - i];                 // U[i][i2]
+        const int rr = i2 * NB - i2 * (i2 - 1) / 2;       // packed (i2, i2)
+#pragma unroll
+        for (int c = i2; c < NB; ++c) {                   // A[i2][c] -= conj(U[i][i2]) U[i][c]
+          const float2 uc = R[ii + c - i];
+          R[rr + c - i2].x -= ui.x * uc.x + ui.y * uc.y;
+          R[rr + c - i2].y -= ui.x * uc.y - ui.y * uc.x;
+        }
+      }
+    }
+  }
+  // forward U^H v = sqrt(L) e_0
+  float2 v[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    float2 acc = make_float2(i == 0 ? a.sqrtL : 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < i; ++k) {                       // acc -= conj(U[k][i]) v_k
+      const float2 u = R[k * NB - k * (k - 1) / 2 + i - k];
+      acc.x -= u.x * v[k].x + u.y * v[k].y;
+      acc.y -= u.x * v[k].y - u.y * v[k].x;
+    }
+    v[i] = make_float2(acc.x * invd[i], acc.y * invd[i]);
+  }
+  float q = 0.f;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) q += v[i].x * v[i].x + v[i].y * v[i].y;
+  const size_t p = (size_t)ix * Nzp + j;
+  if (a.P) a.P[p] = 1.f / q;
+  if (!(a.z || a.wbs || a.w)) return;
+  // back U y = v
+  float2 y[NB];
+#pragma unroll
+  for (int i = NB - 1; i >= 0; --i) {
+    float2 acc = v[i];
+    const int ii = i * NB - i * (i - 1) / 2;
+#pragma unroll
+    for (int c = i + 1; c < NB; ++c) {                  // acc -= U[i][c] y_c
+      const float2 u = R[ii + c - i];
+      acc.x -= u.x * y[c].x - u.y * y[c].y;
+      acc.y -= u.x * y[c].y + u.y * y[c].x;
+    }
+    y[i] = make_float2(acc.x * invd[i], acc.y * invd[i]);
+  }
+  const float iq = 1.f / q;
+  float2 zz = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const float2 wb = make_float2(y[b].x * iq, y[b].y * iq);
+    const float2 s = smean[r + Y][b];
+    zz.x += wb.x * s.x + wb.y * s.y;                    // conj(w) s
+    zz.y += wb.x * s.y - wb.y * s.x;
+    y[b] = wb;
+    if (a.wbs) a.wbs[p * NB + b] = wb;
+  }
+  if (a.z) a.z[p] = zz;
+  if (a.w) {
+    for (int l = 0; l < L; ++l) {                       // w = B^H w_bs
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const float2 tw = a.tw[b * L + l];              // B[b][l] = tw / sqrt(L)
+        acc.x += tw.x * y[b].x + tw.y * y[b].y;
+        acc.y += tw.x * y[b].y - tw.y * y[b].x;
+      }
+      a.w[p * a.wstride + l] = make_float2(acc.x * rsL, acc.y * rsL);
+    }
+  }
+}
+
+template <int NB>
+static hipError_t launch_bs(const BeamspaceLaunch& a, hipStream_t s) {
+  constexpr int RB = NB <= 5 ? 256 : 128;
+  const int TP = RB - 2 * a.Y;
+  if (TP < 1) return hipErrorInvalidValue;
+  const int Nzp = a.Nz - 2 * a.Y;
+  dim3 grid((Nzp + TP - 1) / TP, a.Nx);
+  hipLaunchKernelGGL((beamspace_capon_kernel<NB, RB>), grid, dim3(RB), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_beamspace(const BeamspaceLaunch& a, hipStream_t s) {
+  switch (a.Nb) {
+    case 1: return launch_bs<1>(a, s);
+    case 2: return launch_bs<2>(a, s);
+    case 3: return launch_bs<3>(a, s);
+    case 4: return launch_bs<4>(a, s);
+    case 5: return launch_bs<5>(a, s);
+    case 6: return launch_bs<6>(a, s);
+    case 7: return launch_bs<7>(a, s);
+    case 8: return launch_bs<8>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace cosk

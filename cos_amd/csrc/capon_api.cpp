@@ -15,6 +15,7 @@
 
 #include "../../include/cos_capon.h"
 #include "capon_kernels.h"
+#include "beamspace.h"
 #include "components.h"
 
 namespace {
@@ -52,14 +53,37 @@ struct DevBuf {
 
 }  // namespace
 
+// The frame's launches captured once and replayed: valid while the descriptor, every
+// pointer and wstride are unchanged.
+struct FrameGraph {
+  bool valid = false;
+  cos_capon_desc ds;
+  const void* ptr[7];
+  int wstride = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    valid = false;
+  }
+};
+
 struct cos_capon_plan {
   DevBuf flags, list, count;  // fallback bookkeeping
   DevBuf x, z, w, R, P, B;    // drop-in staging
+  DevBuf bs_tw;               // beamspace twiddles [Nb][L] + [Nb]
+  int bs_Nb = 0, bs_L = 0;
   hipStream_t last_stream = nullptr;
-  int last_fallback = 0;
+  hipStream_t capture_stream = nullptr;  // graphs are captured here, launched on the caller's
+  FrameGraph fg;
   bool fallback_ran = false;
   ~cos_capon_plan() {
-    for (DevBuf* b : {&flags, &list, &count, &x, &z, &w, &R, &P, &B}) b->release();
+    fg.reset();
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+    for (DevBuf* b : {&flags, &list, &count, &x, &z, &w, &R, &P, &B, &bs_tw}) b->release();
   }
 };
 
@@ -147,12 +171,15 @@ int cos_capon_plan_destroy(cos_capon_plan* plan) {
   return 0;
 }
 
-int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x, float* P,
-                  cos_cf32* z, cos_cf32* w, int wstride, cos_cf32* R, unsigned char* flags,
-                  void* stream) {
-  g_last_error.clear();
-  if (!plan || !desc || !x) return fail(hipErrorInvalidValue, "plan, desc and x must be non-NULL");
-  const cos_capon_desc& ds = *desc;
+}  // extern "C"
+
+namespace {
+
+// Enqueue one frame on stream s (arguments already validated; buffers already sized).
+int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32* x, float* P, cos_cf32* z,
+                  cos_cf32* w, int wstride, cos_cf32* R, unsigned char* flags, hipStream_t s, bool fb_needed);
+
+int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
   const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
   if (tables && !(ds.row_L && ds.row_M && ds.row_off && ds.row_d))
     return fail(hipErrorInvalidValue, "row tables must be given together (L, M, off, d)");
@@ -167,8 +194,30 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   if (R && tables) return fail(hipErrorInvalidValue, "R output needs a uniform L");
   if (w && wstride < Lmax) return fail(hipErrorInvalidValue, "wstride < L");
   if (ds.nsteer < 0 || (ds.nsteer > 0 && !ds.steer_sin)) return fail(hipErrorInvalidValue, "bad steering");
-  hipStream_t s = (hipStream_t)stream;
+  if (ds.precision == COS_PREC_FP32_FALLBACK && ds.nsteer > 0)
+    return fail(hipErrorInvalidValue, "FP64 fallback does not support steering");
+  if (ds.precision == COS_PREC_FP32_FALLBACK && Lmax > 32) return fail(hipErrorInvalidValue, "FP64 fallback supports L <= 32");
+  return 0;
+}
 
+// Whether the FP64 fallback pass can find work.  With one loading factor for the frame,
+// cond(R_L) <= 1 + L/d; when that is below the threshold no pixel gets the condition
+// flag, and when (1 + L/d) L 2^-24 < 0.01 an FP32 Cholesky cannot meet a non-positive
+// pivot either (breakdown needs cond ~ 1/(L eps)), so the pass would process no pixel.
+bool fallback_needed(const cos_capon_desc& ds) {
+  if (ds.precision != COS_PREC_FP32_FALLBACK) return false;
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  if (tables || !(ds.d > 0.f)) return true;
+  const double bound = 1.0 + (double)ds.L / (double)ds.d;
+  const double thresh = ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6;
+  return bound > thresh || bound * ds.L * 5.9604645e-8 >= 0.01;
+}
+
+int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32* x, float* P, cos_cf32* z,
+                  cos_cf32* w, int wstride, cos_cf32* R, unsigned char* flags, hipStream_t s, bool fb_needed) {
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  const int Lmax = tables ? ds.Lmax : ds.L;
+  const int Mmax = tables ? ds.Mmax : ds.M;
   cosk::CaponLaunch a = base_launch(ds, x);
   a.P = P;
   a.z = (float2*)z;
@@ -178,27 +227,18 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   a.Rstride = ds.L * ds.L;
   const char* why = nullptr;
   hipError_t e;
-  plan->fallback_ran = false;
-  plan->last_stream = s;
   if (ds.precision == COS_PREC_FP64) {
     e = cosk::launch_capon_f64(a, Lmax, Mmax, s, &why);
     if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 kernel");
     return 0;
   }
-  const bool fb = ds.precision == COS_PREC_FP32_FALLBACK;
-  if (fb && ds.nsteer > 0) return fail(hipErrorInvalidValue, "FP64 fallback does not support steering");
-  if (fb && Lmax > 32) return fail(hipErrorInvalidValue, "FP64 fallback supports L <= 32");
+  const bool fb = fb_needed;
   uint8_t* fl = (uint8_t*)flags;
-  if (fb && !fl) {
-    if ((e = plan->flags.ensure(a.npix)) != hipSuccess) return fail_hip(e, "flags buffer");
-    fl = plan->flags.as<uint8_t>();
-  }
+  if (fb && !fl) fl = plan->flags.as<uint8_t>();
   a.flags = fl;
   e = cosk::launch_capon_f32(a, Lmax, Mmax, s, &why);
   if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
   if (!fb) return 0;
-  if ((e = plan->list.ensure(sizeof(int) * (size_t)a.npix)) != hipSuccess) return fail_hip(e, "list buffer");
-  if ((e = plan->count.ensure(sizeof(int))) != hipSuccess) return fail_hip(e, "count buffer");
   e = cosk::compact_flags(fl, a.npix, plan->list.as<int>(), plan->count.as<int>(), s);
   if (e != hipSuccess) return fail_hip(e, "compaction");
   cosk::CaponLaunch f = a;
@@ -207,7 +247,110 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   f.pcount = plan->count.as<int>();
   e = cosk::launch_capon_f64(f, Lmax, Mmax, s, &why);
   if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 fallback");
-  plan->fallback_ran = true;
+  return 0;
+}
+
+bool same_desc(const cos_capon_desc& a, const cos_capon_desc& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x, float* P,
+                  cos_cf32* z, cos_cf32* w, int wstride, cos_cf32* R, unsigned char* flags,
+                  void* stream) {
+  g_last_error.clear();
+  if (!plan || !desc || !x) return fail(hipErrorInvalidValue, "plan, desc and x must be non-NULL");
+  cos_capon_desc ds = *desc;
+  if (int e = validate(ds, R, w, wstride)) return e;
+  hipStream_t s = (hipStream_t)stream;
+  const bool fb = fallback_needed(ds);
+  const int npix = ds.Nx * (ds.Nz - 2 * ds.Yavg);
+  hipError_t e;
+  if (fb) {
+    if (!flags && (e = plan->flags.ensure(npix)) != hipSuccess) return fail_hip(e, "flags buffer");
+    if ((e = plan->list.ensure(sizeof(int) * (size_t)npix)) != hipSuccess) return fail_hip(e, "list buffer");
+    if ((e = plan->count.ensure(sizeof(int))) != hipSuccess) return fail_hip(e, "count buffer");
+  }
+  plan->last_stream = s;
+  plan->fallback_ran = fb;
+  if (!ds.graph) return enqueue_frame(plan, ds, x, P, z, w, wstride, R, flags, s, fb);
+
+  const void* ptr[7] = {x, P, z, w, R, flags, nullptr};
+  FrameGraph& g = plan->fg;
+  if (!(g.valid && same_desc(g.ds, ds) && std::memcmp(g.ptr, ptr, sizeof(ptr)) == 0 && g.wstride == wstride)) {
+    g.reset();
+    if (!plan->capture_stream && (e = hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking)) != hipSuccess)
+      return fail_hip(e, "capture stream");
+    if ((e = hipStreamBeginCapture(plan->capture_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return fail_hip(e, "begin capture");
+    const int r = enqueue_frame(plan, ds, x, P, z, w, wstride, R, flags, plan->capture_stream, fb);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(plan->capture_stream, &graph);
+    if (r) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    if (e != hipSuccess) return fail_hip(e, "end capture");
+    g.graph = graph;
+    if ((e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0)) != hipSuccess) {
+      g.reset();
+      return fail_hip(e, "graph instantiate");
+    }
+    g.ds = ds;
+    std::memcpy(g.ptr, ptr, sizeof(ptr));
+    g.wstride = wstride;
+    g.valid = true;
+  }
+  if ((e = hipGraphLaunch(g.exec, s)) != hipSuccess) return fail_hip(e, "graph launch");
+  return 0;
+}
+
+int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, int Nb, const cos_cf32* x,
+                            float* P, cos_cf32* z, cos_cf32* w_bs, cos_cf32* w, int wstride, void* stream) {
+  g_last_error.clear();
+  if (!plan || !desc || !x) return fail(hipErrorInvalidValue, "plan, desc and x must be non-NULL");
+  const cos_capon_desc& ds = *desc;
+  if (ds.row_L || ds.row_M || ds.row_off || ds.row_d || ds.nsteer)
+    return fail(hipErrorInvalidValue, "beamspace path takes uniform L, M, d and no steering");
+  if (int e = check_args(ds.d, ds.L, ds.Yavg, ds.M, ds.Nx, ds.Nz, Nb)) return e;
+  if (Nb < 1 || Nb > 8) return fail(hipErrorInvalidValue, "beamspace path supports 1 <= Nb <= 8");
+  if (ds.off < 0 || ds.off + ds.M > ds.Mstride) return fail(hipErrorInvalidValue, "off + M > Mstride");
+  if (2 * ds.Yavg > 120) return fail(hipErrorInvalidValue, "beamspace path supports Yavg <= 60");
+  if (w && wstride < ds.L) return fail(hipErrorInvalidValue, "wstride < L");
+  hipError_t e;
+  if (plan->bs_Nb != Nb || plan->bs_L != ds.L) {
+    // beams beta(b) (RealTimeCapon/capon_kernel.cu:53-60 without the never-firing even-Nb +1)
+    std::vector<float> h(2 * ((size_t)Nb * ds.L + Nb));
+    const int L = ds.L;
+    for (int b = 0; b < Nb; ++b) {
+      const int beta = b > Nb / 2 ? b + L - 2 * (Nb / 2) - 1 : b;
+      for (int l = 0; l < L; ++l) {
+        const double ph = -2.0 * M_PI * (double)((long)beta * l % L) / (double)L;
+        h[2 * ((size_t)b * L + l)] = (float)std::cos(ph);
+        h[2 * ((size_t)b * L + l) + 1] = (float)std::sin(ph);
+      }
+      const double ph = 2.0 * M_PI * (double)beta / (double)L;
+      h[2 * ((size_t)Nb * L + b)] = (float)std::cos(ph);
+      h[2 * ((size_t)Nb * L + b) + 1] = (float)std::sin(ph);
+    }
+    if ((e = plan->bs_tw.ensure(h.size() * sizeof(float))) != hipSuccess) return fail_hip(e, "twiddle buffer");
+    if ((e = hipMemcpy(plan->bs_tw.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail_hip(e, "copy twiddles");
+    plan->bs_Nb = Nb;
+    plan->bs_L = ds.L;
+  }
+  cosk::BeamspaceLaunch a;
+  std::memset(&a, 0, sizeof(a));
+  a.x = (const float2*)x;
+  a.Nx = ds.Nx; a.Nz = ds.Nz; a.Mstride = ds.Mstride; a.M = ds.M; a.L = ds.L; a.off = ds.off;
+  a.Y = ds.Yavg; a.Nb = Nb; a.d = ds.d;
+  a.rsqrtL = (float)(1.0 / std::sqrt((double)ds.L));
+  a.sqrtL = (float)std::sqrt((double)ds.L);
+  a.tw = plan->bs_tw.as<float2>();
+  a.omega = plan->bs_tw.as<float2>() + (size_t)Nb * ds.L;
+  a.P = P; a.z = (float2*)z; a.wbs = (float2*)w_bs; a.w = (float2*)w; a.wstride = wstride;
+  if ((e = cosk::launch_beamspace(a, (hipStream_t)stream)) != hipSuccess) return fail_hip(e, "beamspace kernel");
   return 0;
 }
 

@@ -170,3 +170,63 @@ def config5_frames(Nx: int = 128, Nz: int = 512, M: int = 64, n_frames: int = 8,
 
 def config5_phase(M: int = 64, seed: int = 5) -> np.ndarray:
     return (uniforms(seed, 0, M) - 0.5).astype(np.float32)
+
+
+# ------------------------------------------------------------------------------------
+# The same generator on a torch device (used for cubes too large for numpy).  int64
+# arithmetic wraps like uint64; right shifts are made logical by masking.
+# ------------------------------------------------------------------------------------
+
+def _t_mix(z):
+    import torch
+    m30 = (1 << 34) - 1
+    m27 = (1 << 37) - 1
+    m31 = (1 << 33) - 1
+    c1 = int(np.array([0xBF58476D1CE4E5B9], dtype=np.uint64).view(np.int64)[0])
+    c2 = int(np.array([0x94D049BB133111EB], dtype=np.uint64).view(np.int64)[0])
+    z = z ^ ((z >> 30) & m30)
+    z = z * c1
+    z = z ^ ((z >> 27) & m27)
+    z = z * c2
+    return z ^ ((z >> 31) & m31)
+
+
+def cnormal_torch(seed: int, stream: int, shape, device, chunk: int = 1 << 24):
+    """Same numbers as cnormal() (complex128 arithmetic, returned as complex64) computed
+    with torch on `device`, in chunks."""
+    import torch
+    n = int(np.prod(shape))
+    key = int(np.array([_key(seed, stream)], dtype=np.uint64).view(np.int64)[0])
+    gold = int(np.array([GOLDEN], dtype=np.uint64).view(np.int64)[0])
+    out = torch.empty(n, dtype=torch.complex64, device=device)
+    m53 = (1 << 53) - 1
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        ctr = torch.arange(2 * s + 1, 2 * e + 1, dtype=torch.int64, device=device)
+        z = _t_mix(key + ctr * gold)
+        u = ((z >> 11) & m53).to(torch.float64) * (1.0 / 9007199254740992.0)
+        r = torch.sqrt(-torch.log1p(-u[0::2]))
+        th = 2.0 * np.pi * u[1::2]
+        out[s:e] = torch.complex(r * torch.cos(th), r * torch.sin(th)).to(torch.complex64)
+        del ctr, z, u, r, th
+    return out.reshape(shape)
+
+
+def config4_torch(Nx: int = 512, Nz: int = 8192, M: int = 96, seed: int = 4, device="cuda",
+                  n_sources: int = 4, span: int = 200):
+    """config4() generated on a torch device: identical recipe (noise from the counter
+    generator, the same seeded sources), returned as a complex64 tensor."""
+    import torch
+    x = cnormal_torch(seed, 0, (Nx, Nz, M), device)
+    u = uniforms(seed, 1, 4 * n_sources)
+    m = torch.arange(M, dtype=torch.float64, device=device)
+    looks = torch.linspace(-0.5, 0.5, Nx, dtype=torch.float64, device=device)
+    for s in range(n_sources):
+        th = np.deg2rad(-30.0 + 60.0 * u[4 * s])
+        snr = 10.0 + 15.0 * u[4 * s + 1]
+        z0 = int(u[4 * s + 2] * (Nz - span))
+        amp = np.sqrt(10.0 ** (snr / 10.0))
+        sig = torch.from_numpy(cnormal(seed, 10 + s, (span,)) * amp).to(device)
+        ph = torch.exp(1j * np.pi * m[None, :] * (np.sin(th) - looks[:, None]))       # [Nx][M]
+        x[:, z0:z0 + span, :] += (sig[None, :, None] * ph[:, None, :]).to(torch.complex64)
+    return x

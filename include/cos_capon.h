@@ -92,6 +92,8 @@ typedef struct {
     float cond_thresh;   /* fallback when 1 + L/d exceeds this (default 1e6 when <= 0) */
     const float* steer_sin; /* optional device array of nsteer sin(theta) values */
     int nsteer;
+    int graph;           /* nonzero: capture the frame's launches in a hipGraph on the first
+                            call and replay it while descriptor and pointers are unchanged */
 } cos_capon_desc;
 
 int cos_capon_plan_create(cos_capon_plan** plan);
@@ -105,6 +107,17 @@ int cos_capon_plan_destroy(cos_capon_plan* plan);
 int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x,
                   float* P, cos_cf32* z, cos_cf32* w, int wstride, cos_cf32* R,
                   unsigned char* flags, void* stream);
+
+/* Beamspace Capon with the sliding DFT: replaces the reference's SLIDING_BEAMSPACE build
+ * (RealTimeCapon/sliding_beamspace_kernel.cu:26-237 -> build_R_full -> Nb-solve ->
+ * getCaponOutputFromSubarraySums, RealTimeCapon/Capon.cpp:202-227,399-415) with loading
+ * d*tr(R_bs)/Nb as in framework/beamformer/capon/getCaponAllC.pyx:376-385.  Uses Nx, Nz,
+ * Mstride, M, L, off, Yavg, d of desc; 1 <= Nb <= 8 Butler beams symmetric about broadside.
+ * Outputs (device, any may be NULL): P [npx] power 1/(a_bs^H R_bs^-1 a_bs), z [npx]
+ * amplitude w_bs^H (B g), w_bs [npx][Nb], w [npx][wstride] element-space weights B^H w_bs. */
+int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, int Nb,
+                            const cos_cf32* x, float* P, cos_cf32* z, cos_cf32* w_bs,
+                            cos_cf32* w, int wstride, void* stream);
 
 /* Number of pixels the last cos_capon_run sent to the FP64 fallback (synchronises the
  * plan's stream). */

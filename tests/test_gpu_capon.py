@@ -271,3 +271,85 @@ def test_full_frame_properties():
     sub = c_capon(x, 0.01, 32, 2, ix=(0, 256), j=(1000, 1010), want_w=False)
     Pn = P.reshape(256, 2044)[:, 1000:1010]
     assert rel(Pn, sub["P"]).max() <= 1e-4
+
+
+def test_graph_replay_matches_eager():
+    """The frame captured in a hipGraph and replayed (with eager work and synchronisation
+    between replays) gives the eager result every time; new pointers re-capture."""
+    import torch
+    from cos_amd import CaponPlan, synth
+    say("graph replay")
+    x = torch.from_numpy(synth.config2(Nx=8, Nz=256, M=64, seed=2)).cuda()
+    plan = CaponPlan()
+    eager = plan.run(x, 0.01, 32, 2, precision="fp32+fallback")
+    torch.cuda.synchronize()
+    ref = {k: v.clone() for k, v in eager.items()}
+    out = {k: torch.full_like(v, float("nan")) if v.is_floating_point() or v.is_complex() else torch.zeros_like(v)
+           for k, v in eager.items()}
+    for rep in range(4):
+        for v in out.values():
+            v.zero_()
+        plan.run(x, 0.01, 32, 2, precision="fp32+fallback", out=out, graph=True)
+        torch.cuda.synchronize()
+        _ = (x.abs().sum() * 2).item()                      # eager work between replays
+        for k in ref:
+            assert torch.equal(out[k], ref[k]), (rep, k)
+    # different output tensors -> a new capture, same values
+    out2 = {k: torch.empty_like(v) for k, v in eager.items()}
+    plan.run(x, 0.01, 32, 2, precision="fp32+fallback", out=out2, graph=True)
+    torch.cuda.synchronize()
+    for k in ref:
+        assert torch.equal(out2[k], ref[k]), k
+    # rows with tiny loading: the fallback pass runs inside the graph
+    rows = {"L": torch.full((256,), 32, dtype=torch.int32).cuda(), "M": torch.full((256,), 64, dtype=torch.int32).cuda(),
+            "off": torch.zeros(256, dtype=torch.int32).cuda(), "d": torch.full((256,), 1e-6).cuda()}
+    e2 = plan.run(x, 0.0, 0, 2, rows=rows, precision="fp32+fallback")
+    torch.cuda.synchronize()
+    n_eager = plan.fallback_count()
+    e2r = {k: v.clone() for k, v in e2.items()}
+    o3 = {k: torch.empty_like(v) for k, v in e2.items()}
+    for rep in range(3):
+        plan.run(x, 0.0, 0, 2, rows=rows, precision="fp32+fallback", out=o3, graph=True)
+        torch.cuda.synchronize()
+        assert plan.fallback_count() == n_eager == 8 * 252
+        for k in e2r:
+            assert torch.equal(o3[k], e2r[k]), (rep, k)
+
+
+# ------------------------------------------------------------------ sliding-DFT beamspace
+
+@pytest.mark.parametrize("M,L,Y,Nb,d", [(12, 6, 1, 2, 0.2), (32, 16, 2, 5, 0.05), (20, 9, 0, 1, 0.1),
+                                        (40, 20, 3, 8, 0.01), (16, 8, 1, 4, 0.1)])
+def test_beamspace_sliding_matches_oracle(M, L, Y, Nb, d):
+    import torch
+    from cos_amd import CaponPlan
+    say("beamspace sliding", M, L, Y, Nb, d)
+    x = uniform_cube(300 + Nb + M, 3, 40, M)
+    o = CaponPlan().run_beamspace(torch.from_numpy(x).cuda(), d, L, Y, Nb, want=("P", "z", "w_bs", "w"))
+    torch.cuda.synchronize()
+    ref = O.capon_beamspace(x.astype(np.complex128), d, L, Y, Nb)
+    P = o["P"].cpu().numpy()
+    assert rel(P, ref["P"]).max() <= 1e-4
+    z = o["z"].cpu().numpy()
+    assert np.abs(z - ref["z"]).max() <= 1e-4 * np.abs(ref["z"]).max()
+    wbs = o["w_bs"].cpu().numpy()
+    assert np.abs(wbs - ref["w_bs"]).max() <= 1e-4 * np.abs(ref["w_bs"]).max()
+    w = o["w"].cpu().numpy()
+    assert np.abs(w - ref["w"]).max() <= 1e-4 * np.abs(ref["w"]).max()
+
+
+def test_beamspace_sonar_slice():
+    """The 96-element, L = 48, Nb = 5, Yavg = 4 sonar geometry on 8 look directions."""
+    import torch
+    from cos_amd import CaponPlan, synth
+    say("beamspace sonar slice", 8, 1024, 96, 48, 5)
+    x = synth.config4(Nx=8, Nz=1024, M=96, seed=4)
+    o = CaponPlan().run_beamspace(torch.from_numpy(x).cuda(), 0.01, 48, 4, 5)
+    torch.cuda.synchronize()
+    P = o["P"].cpu().numpy()
+    ix, j = O.all_pixels(8, 1024, 4)
+    sel = np.arange(0, len(ix), 7)
+    ref = O.capon_beamspace(x.astype(np.complex128), 0.01, 48, 4, 5, pixels=(ix[sel], j[sel]))
+    e = rel(P[sel], ref["P"])
+    print(f"sonar P rel err max {e.max():.3e}", flush=True)
+    assert e.max() <= 1e-4
