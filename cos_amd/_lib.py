@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libcos_capon.so")
 EXPORTS = (
     "cos_capon_get_capon", "cos_capon_get_capon_p", "cos_buildr_get_r", "cos_solver_solve",
     "cos_capon_last_error", "cos_capon_plan_create", "cos_capon_plan_destroy", "cos_capon_run",
-    "cos_capon_last_fallback_count", "cos_capon_beamspace_run",
+    "cos_capon_last_fallback_count", "cos_capon_beamspace_run", "cos_capon_backward",
 )
 
 COS_PREC_FP32 = 0
@@ -72,6 +72,7 @@ def lib() -> C.CDLL:
     L.cos_capon_run.argtypes = [vp, C.POINTER(CaponDesc), vp, vp, vp, vp, ip, vp, vp, vp]
     L.cos_capon_last_fallback_count.argtypes = [vp, C.POINTER(C.c_int)]
     L.cos_capon_beamspace_run.argtypes = [vp, C.POINTER(CaponDesc), ip, vp, vp, vp, vp, vp, ip, vp]
+    L.cos_capon_backward.argtypes = [vp, C.POINTER(CaponDesc), vp, vp, vp, vp, vp]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.cos_capon_last_error.restype = C.c_char_p

@@ -354,6 +354,27 @@ int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, in
   return 0;
 }
 
+int cos_capon_backward(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x,
+                       const float* grad_P, cos_cf32* grad_x, float* grad_d, void* stream) {
+  g_last_error.clear();
+  if (!plan || !desc || !x || !grad_P || !grad_x) return fail(hipErrorInvalidValue, "NULL argument");
+  cos_capon_desc ds = *desc;
+  ds.graph = 0;
+  if (ds.nsteer) return fail(hipErrorInvalidValue, "the adjoint does not take steering vectors");
+  if (int e = validate(ds, nullptr, nullptr, 0)) return e;
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  const int Lmax = tables ? ds.Lmax : ds.L;
+  const int Mmax = tables ? ds.Mmax : ds.M;
+  cosk::CaponLaunch a = base_launch(ds, x);
+  a.gP = grad_P;
+  a.gx = (float2*)grad_x;
+  a.gd = grad_d;
+  const char* why = nullptr;
+  hipError_t e = cosk::launch_capon_bwd(a, Lmax, Mmax, (hipStream_t)stream, &why);
+  if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "adjoint kernel");
+  return 0;
+}
+
 int cos_capon_last_fallback_count(cos_capon_plan* plan, int* count) {
   if (!plan || !count) return fail(hipErrorInvalidValue, "NULL argument");
   *count = 0;

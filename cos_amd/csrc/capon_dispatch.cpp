@@ -3,15 +3,15 @@
 
 namespace cosk {
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, bool BWD>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s);
 
 template <int G, int CPL>
 static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
   const int TW = 2 * a.Y + 1;
-  if (TW <= 5) return launch_one<float, G, CPL, 5, false, false>(a, Mmax, s);
-  if (TW <= 9) return launch_one<float, G, CPL, 9, false, false>(a, Mmax, s);
-  return launch_one<float, G, CPL, 0, false, false>(a, Mmax, s);
+  if (TW <= 5) return launch_one<float, G, CPL, 5, false, false, false>(a, Mmax, s);
+  if (TW <= 9) return launch_one<float, G, CPL, 9, false, false, false>(a, Mmax, s);
+  return launch_one<float, G, CPL, 0, false, false, false>(a, Mmax, s);
 }
 
 hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why) {
@@ -29,10 +29,24 @@ hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_
   }
   const bool list = a.pcount != nullptr;
   if (Lmax <= 16)
-    return list ? launch_one<double, 16, 1, 0, true, true>(a, Mmax, s) : launch_one<double, 16, 1, 0, true, false>(a, Mmax, s);
+    return list ? launch_one<double, 16, 1, 0, true, true, false>(a, Mmax, s) : launch_one<double, 16, 1, 0, true, false, false>(a, Mmax, s);
   if (Lmax <= 32)
-    return list ? launch_one<double, 16, 2, 0, true, true>(a, Mmax, s) : launch_one<double, 16, 2, 0, true, false>(a, Mmax, s);
+    return list ? launch_one<double, 16, 2, 0, true, true, false>(a, Mmax, s) : launch_one<double, 16, 2, 0, true, false, false>(a, Mmax, s);
   if (why) *why = "L > 32 is not supported by the FP64 kernel";
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_capon_bwd(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why) {
+  if (a.nsteer > 0) {
+    if (why) *why = "the adjoint does not take steering vectors";
+    return hipErrorInvalidValue;
+  }
+  const bool w5 = 2 * a.Y + 1 <= 5;
+  if (Lmax <= 16)
+    return w5 ? launch_one<float, 16, 1, 5, false, false, true>(a, Mmax, s) : launch_one<float, 16, 1, 0, false, false, true>(a, Mmax, s);
+  if (Lmax <= 32)
+    return w5 ? launch_one<float, 16, 2, 5, false, false, true>(a, Mmax, s) : launch_one<float, 16, 2, 0, false, false, true>(a, Mmax, s);
+  if (why) *why = "the adjoint supports L <= 32";
   return hipErrorInvalidValue;
 }
 

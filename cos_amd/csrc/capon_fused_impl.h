@@ -44,6 +44,9 @@ __device__ __forceinline__ cplx<T> ldx(const float2* p) {
   return {(T)v.x, (T)v.y};
 }
 
+// Largest K = M - L + 1 the adjoint epilogue handles (its per-lane k and m slots).
+constexpr int BWD_KMAX = 128;
+
 template <int G, int S, int R>
 __device__ __forceinline__ constexpr int slot() { return G * S * (S + 1) / 2 + R; }
 
@@ -79,7 +82,7 @@ template <typename T, int G, int CPL> struct OccHint {
   static constexpr int value = sizeof(T) == 4 ? (G == 16 ? (CPL == 1 ? 4 : 2) : 1) : (CPL == 1 ? 2 : 1);
 };
 
-template <typename T, int G, int CPL, int TM, bool DIRECT>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool BWD = false>
 __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, const int GREG,
                                             const int gid0, const int npix) {
   using Gp = Grp<G>;
@@ -383,7 +386,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // Strictly upper U[r][c] (r < c) packed at r + c(c+1)/2 into this group's LDS region,
   // which held the snapshots (all reads of them are complete: the values above depend
   // on them).  The back solve reads it row-wise.
-  const bool want_y = a.w || a.z;
+  const bool want_y = a.w || a.z || BWD;
   cplx<T>* Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
   if (want_y) {
     static_for<0, CPL>([&](auto S_) {
@@ -487,6 +490,88 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       zp.y = Gp::sum(zp.y);
       if (active && g == 0 && a.z)
         a.z[po] = valid ? make_float2((float)zp.x, (float)zp.y) : make_float2(nanv, nanv);
+
+      if constexpr (BWD) {
+        // Adjoint of P = 1/(norm q), q = a^H S_L^-1 a, S_L = S + (d tr S / L) I:
+        //   dq = -y^H dS y - |y|^2 d(d tr S/L),  y = S_L^-1 a
+        //   dq/dx_t[m] = -2 sum_k (Gamma u_{t,k})[m-k],  Gamma = y y^H + (d |y|^2 / L) I
+        //              = -2 (sum_l y_l beta_{t,m-l} + (d |y|^2/L) cnt(m) x_t[m]),
+        //   beta_{t,k} = y^H u_{t,k};   dq/dd = -tr S |y|^2 / L.
+        // Gradients use the convention g = d/dRe + j d/dIm (torch's for real losses).
+        T yn = T(0);
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          yn = fma(y[S].x, y[S].x, fma(y[S].y, y[S].y, yn));
+        });
+        yn = Gp::sum(yn);
+        const T gPp = (active && valid) ? (T)a.gP[p] : T(0);
+        const T coeff = -gPp / (norm * accP * accP);            // dl/dq
+        const T cload = (T)dload * yn / (T)L;
+        if (active && g == 0 && a.gd) a.gd[p] = (float)(coeff * (-trS * yn / (T)L));
+        // the window rows again (the group's region held U), then beta after them
+        float2* xr0 = xs_all + q * GREG;
+        cplx<T>* beta = reinterpret_cast<cplx<T>*>(xr0 + TW * MP);
+        const float2* srcw = a.x + ((size_t)ix * a.Nz + j) * a.Mstride + off;
+        for (int t = 0; t < TW; ++t)
+          for (int m = g; m < MP; m += G)
+            xr0[t * MP + m] = (m < M) ? srcw[(size_t)t * a.Mstride + m] : make_float2(0.f, 0.f);
+        __syncthreads();
+        constexpr int KS = (BWD_KMAX + G - 1) / G;
+        for (int t = 0; t < TW; ++t) {
+          const float2* xt = xr0 + t * MP;
+          cplx<T> bacc[KS];
+          static_for<0, KS>([&](auto S_) {
+            constexpr int S = decltype(S_)::value;
+            bacc[S] = {T(0), T(0)};
+          });
+          static_for<0, LMAX>([&](auto l_) {
+            constexpr int l = decltype(l_)::value;
+            const cplx<T> yb = bcastc<G, l % G>(y[l / G]);
+            static_for<0, KS>([&](auto S_) {
+              constexpr int S = decltype(S_)::value;
+              const int k = g + G * S;
+              if (k < K) add_conj_mul(bacc[S], yb, ldx<T>(xt + k + l));
+            });
+          });
+          static_for<0, KS>([&](auto S_) {
+            constexpr int S = decltype(S_)::value;
+            const int k = g + G * S;
+            if (k < K) beta[k] = bacc[S];
+          });
+          __syncthreads();
+          cplx<T> gacc[KS];
+          static_for<0, KS>([&](auto S_) {
+            constexpr int S = decltype(S_)::value;
+            gacc[S] = {T(0), T(0)};
+          });
+          static_for<0, LMAX>([&](auto l_) {
+            constexpr int l = decltype(l_)::value;
+            const cplx<T> yb = bcastc<G, l % G>(y[l / G]);
+            static_for<0, KS>([&](auto S_) {
+              constexpr int S = decltype(S_)::value;
+              const int m = g + G * S, kk = m - l;
+              if (m < M && kk >= 0 && kk < K) add_mul(gacc[S], yb, beta[kk]);
+            });
+          });
+          if (active && valid) {
+            static_for<0, KS>([&](auto S_) {
+              constexpr int S = decltype(S_)::value;
+              const int m = g + G * S;
+              if (m < M) {
+                const int cnt = min(L - 1, m) - max(0, m - K + 1) + 1;
+                const cplx<T> xv = ldx<T>(xt + m);
+                const T f = T(-2) * coeff;
+                const T vx = f * (gacc[S].x + cload * (T)cnt * xv.x);
+                const T vy = f * (gacc[S].y + cload * (T)cnt * xv.y);
+                float* dst = reinterpret_cast<float*>(a.gx + ((size_t)ix * a.Nz + j + t) * a.Mstride + off + m);
+                atomicAdd(dst, (float)vx);
+                atomicAdd(dst + 1, (float)vy);
+              }
+            });
+          }
+          __syncthreads();
+        }
+      }
     }
   }
 
@@ -499,9 +584,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   }
 }
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, bool BWD = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
-                          amdgpu_waves_per_eu(OccHint<T, G, CPL>::value)))
+                          amdgpu_waves_per_eu(BWD ? 1 : OccHint<T, G, CPL>::value)))
 void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   // PERSIST: grid-stride over groups of NG pixels, so a launch over a device-side pixel
   // list (the FP64 fallback) can use a bounded grid; otherwise one block per group (a
@@ -510,30 +595,35 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   const int npix = a.pcount ? min(*a.pcount, a.npix) : a.npix;
   if constexpr (PERSIST) {
     for (int gid0 = blockIdx.x * NGk; gid0 < npix; gid0 += gridDim.x * NGk)
-      capon_group<T, G, CPL, TM, DIRECT>(a, MP, GREG, gid0, npix);
+      capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
   } else {
     // one group of pixels per block (a loop here costs the FP32 kernel its registers)
     const int gid0 = blockIdx.x * NGk;
-    if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT>(a, MP, GREG, gid0, npix);
+    if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
   }
 }
 
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, bool BWD = false>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   constexpr int NG = 64 / G, LMAX = G * CPL;
   const int MP = ((Mmax + LMAX + 4 + 3) / 4) * 4;   // zero padding covers every lag read
   const int TW = 2 * a.Y + 1;
   // per-group LDS region: the window's snapshots, later the packed upper U
   const int upk = LMAX * (LMAX + 1) / 2 * (int)(sizeof(cplx<T>) / sizeof(float2));
-  const int GREG = TW * MP > upk ? TW * MP : upk;
+  int GREG = TW * MP > upk ? TW * MP : upk;
+  if (BWD) {   // window rows plus beta[K <= BWD_KMAX] for the adjoint epilogue
+    if (Mmax > BWD_KMAX) return hipErrorInvalidValue;
+    const int need = TW * MP + BWD_KMAX * (int)(sizeof(cplx<T>) / sizeof(float2));
+    GREG = GREG > need ? GREG : need;
+  }
   const size_t lds = (size_t)NG * GREG * sizeof(float2);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   if (!a.x) return hipErrorInvalidValue;
   int blocks = (a.npix + NG - 1) / NG;
   if (PERSIST && blocks > 2048) blocks = 2048;   // list length known only on the device
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST>), dim3(blocks), dim3(64), lds, s, a, MP, GREG);
+  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST, BWD>), dim3(blocks), dim3(64), lds, s, a, MP, GREG);
   return hipGetLastError();
 }
 

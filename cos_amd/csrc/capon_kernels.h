@@ -31,6 +31,11 @@ struct CaponLaunch {
   float cond_thresh;
   int refine;          // FP32 path: one step of iterative refinement of y
   int cov_only;        // stop after the (optional) R output: BuildR::getR
+  // adjoint (backward launch only): upstream dl/dP [npix], accumulated dl/dx (cube layout,
+  // atomics; caller zeroes it), per-pixel dl/dd [npix]
+  const float* gP;
+  float2* gx;
+  float* gd;
   // steering (optional): nsteer sines; outputs P/z are [p][nsteer], w is [p][nsteer][wstride]
   const float* steer_sin;
   int nsteer;
@@ -39,5 +44,6 @@ struct CaponLaunch {
 // Returns hipSuccess or an error; `why` receives a static message on argument errors.
 hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
 hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
+hipError_t launch_capon_bwd(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
 
 }  // namespace cosk

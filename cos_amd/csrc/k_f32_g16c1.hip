@@ -3,7 +3,7 @@
 #include "capon_fused_impl.h"
 
 namespace cosk {
-template hipError_t launch_one<float, 16, 1, 5, false, false>(const CaponLaunch&, int, hipStream_t);
-template hipError_t launch_one<float, 16, 1, 9, false, false>(const CaponLaunch&, int, hipStream_t);
-template hipError_t launch_one<float, 16, 1, 0, false, false>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<float, 16, 1, 5, false, false, false>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<float, 16, 1, 9, false, false, false>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<float, 16, 1, 0, false, false, false>(const CaponLaunch&, int, hipStream_t);
 }  // namespace cosk

@@ -2,5 +2,5 @@
 #include "capon_fused_impl.h"
 
 namespace cosk {
-template hipError_t launch_one<float, 64, 1, 5, false, false>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<float, 64, 1, 5, false, false, false>(const CaponLaunch&, int, hipStream_t);
 }  // namespace cosk

@@ -3,6 +3,6 @@
 #include "capon_fused_impl.h"
 
 namespace cosk {
-template hipError_t launch_one<double, 16, 2, 0, true, false>(const CaponLaunch&, int, hipStream_t);
-template hipError_t launch_one<double, 16, 2, 0, true, true>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<double, 16, 2, 0, true, false, false>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<double, 16, 2, 0, true, true, false>(const CaponLaunch&, int, hipStream_t);
 }  // namespace cosk

@@ -119,6 +119,15 @@ int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, in
                             const cos_cf32* x, float* P, cos_cf32* z, cos_cf32* w_bs,
                             cos_cf32* w, int wstride, void* stream);
 
+/* Adjoint of the power image (no reference counterpart; the differentiable workload).
+ * Given dl/dP (grad_P [npx], device), adds dl/dx into grad_x (same layout as x; the caller
+ * zeroes it first; accumulated with atomics) and writes the per-pixel dl/dd into grad_d
+ * [npx] (may be NULL; sum it for a frame-wide d, or per row for row tables).  The
+ * covariance and its Cholesky factor are recomputed, not stored.  Gradients follow the
+ * d/dRe + j d/dIm convention.  FP32, L <= 32, M <= 128, no steering. */
+int cos_capon_backward(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x,
+                       const float* grad_P, cos_cf32* grad_x, float* grad_d, void* stream);
+
 /* Number of pixels the last cos_capon_run sent to the FP64 fallback (synchronises the
  * plan's stream). */
 int cos_capon_last_fallback_count(cos_capon_plan* plan, int* count);
