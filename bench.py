@@ -78,11 +78,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparator")
+    ap.add_argument("--workload", default="frame",
+                    choices=["frame", "aperture", "beamspace", "steering", "adjoint"],
+                    help="frame: the headline 256x2048x64 L=32 frame (default); the others time the "
+                         "expanding-aperture, sonar beamspace, steered and differentiable workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); pixels shard by beam "
               "with no exchange, so N GPUs would each run this frame on Nx/N beams.", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "frame":
+        import bench_workloads
+        print(json.dumps(getattr(bench_workloads, args.workload)(args)), flush=True)
+        return
 
     import torch
     from cos_amd import CaponPlan, synth

@@ -1,0 +1,179 @@
+"""Secondary workloads for bench.py (``--workload``): one JSON line each.
+
+aperture   expanding aperture, 256 x 4096, M(z) 16..128, L(z) = M(z)/2, 5% of rows at
+           d = 1e-5 (FP32 + FP64 fallback, against an all-FP64 run of the same frame)
+beamspace  sonar beamspace, 512 look directions x 8192 ranges x 96 elements, L = 48,
+           Nb = 5, Yavg = 4, power image (sliding-DFT kernel)
+steering   far-field ULA, 1024 range bins x 32 elements, L = 16, 128 steering angles
+adjoint    differentiable frames: 8 x (128 x 512 x 64) with per-element phase errors,
+           loss = -mean log P, forward + adjoint
+
+Inputs are generated on the GPU with the counter-based generator (cos_amd.synth), so
+they are resident in HBM before timing starts.  The oracle is used only for the accuracy
+samples.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def _timed(torch, fn, n, stream=None):
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(n):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / n
+
+
+def _flops(L, K):
+    return 8 * L * L * K + 4 * L * L + (4 * L ** 3) // 3 + 8 * L * L + 32 * L
+
+
+def _base(args, metric, value, unit, ms, cfg, dtype="f32"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded)", "config": cfg}
+
+
+def aperture(args):
+    import torch
+    from cos_amd import CaponPlan, synth
+    from oracle import capon_oracle as O
+    Nx, Nz, Mmax, Y = 256, 4096, 128, 2
+    x, Mz, Lz, off, dz = synth.config3_torch(Nx, Nz, seed=3, Mmax=Mmax, device="cuda")
+    rows = {k: torch.from_numpy(v).cuda() for k, v in dict(L=Lz, M=Mz, off=off, d=dz).items()}
+    plan = CaponPlan()
+    npx = Nx * (Nz - 2 * Y)
+    out = {"P": torch.empty(npx, device="cuda"), "z": torch.empty(npx, dtype=torch.complex64, device="cuda"),
+           "w": torch.empty((npx, 64), dtype=torch.complex64, device="cuda")}
+    step = lambda: plan.run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", out=out)
+    for _ in range(args.warmup):
+        step()
+    ms = _timed(torch, step, args.steps)
+    nfb = plan.fallback_count()
+    P32 = out["P"].clone()
+    o64 = {"P": torch.empty(npx, device="cuda")}
+    ms64 = _timed(torch, lambda: plan.run(x, 0.0, 0, Y, rows=rows, precision="fp64", out=o64, want=("P",)), 1)
+    e = (P32.double() - o64["P"].double()).abs() / o64["P"].double()
+    c = np.arange(Nz - 2 * Y) + Y
+    fpx = float(np.mean([_flops(int(Lz[cc]), int(Mz[cc] - Lz[cc] + 1)) for cc in c]))
+    # oracle sample: 4 beams x 16 rows spread over the depth (FP64 numpy)
+    rng = np.random.default_rng(3)
+    beams = np.sort(rng.choice(Nx, 4, replace=False))
+    ib, jj = np.repeat(np.arange(4), 16), rng.integers(0, Nz - 2 * Y, 64)
+    xs = x[torch.from_numpy(beams).cuda()].cpu().numpy().astype(np.complex128)
+    ref = O.capon_rows(xs, dz.astype(np.float64), Lz, Mz, off, Y, pixels=(ib, jj))
+    Ps = P32.cpu().numpy().reshape(Nx, Nz - 2 * Y)[beams[ib], jj]
+    eo = np.abs(Ps - ref["P"]) / ref["P"]
+    line = _base(args, "Capon pixels/sec, expanding aperture M(z)=16..128, L(z)=M(z)/2, FP32 + FP64 fallback",
+                 npx / (ms * 1e-3), "pixels/s", ms,
+                 {"workload": "expanding-aperture frame", "Nx": Nx, "Nz": Nz, "Mmax": Mmax, "Yavg": Y,
+                  "pixels": npx, "ill_rows_d": 1e-5, "fallback_pixels": nfb})
+    line["roofline"] = {"bound": "mfma", "achieved": round(fpx * npx / (ms * 1e-3) / 1e12, 3),
+                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(fpx * npx / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                        "flop_per_pixel_mean": round(fpx)}
+    line["all_fp64_ms_per_frame"] = round(ms64, 3)
+    line["speedup_vs_all_fp64"] = round(ms64 / ms, 2)
+    line["accuracy"] = {"vs_all_fp64_P_rel_err_max": float(e.max()),
+                        "vs_all_fp64_P_rel_err_p999": float(torch.quantile(e[::97].float(), 0.999)),
+                        "vs_oracle_P_rel_err_max_64px": float(eo.max())}
+    return line
+
+
+def beamspace(args):
+    import torch
+    from cos_amd import CaponPlan, synth
+    from oracle import capon_oracle as O
+    Nx, Nz, M, L, Y, Nb, d = 512, 8192, 96, 48, 4, 5, 0.01
+    x = synth.config4_torch(Nx, Nz, M, seed=4, device="cuda")
+    plan = CaponPlan()
+    npx = Nx * (Nz - 2 * Y)
+    out = {"P": torch.empty(npx, device="cuda")}
+    step = lambda: plan.run_beamspace(x, d, L, Y, Nb, want=("P",), out=out)
+    for _ in range(args.warmup):
+        step()
+    ms = _timed(torch, step, args.steps)
+    K = M - L + 1
+    fpx = 8 * Nb * L + 10 * Nb * (K - 1) + 8 * Nb * Nb * K + 4 * Nb * Nb + (4 * Nb ** 3) // 3 + 8 * Nb * Nb + 32 * Nb
+    bpx = 8 * M + 4
+    rng = np.random.default_rng(4)
+    ix, jj = rng.integers(0, Nx, 128), rng.integers(0, Nz - 2 * Y, 128)
+    win = np.stack([x[i, j:j + 2 * Y + 1, :].cpu().numpy() for i, j in zip(ix, jj)])
+    ref = O.capon_beamspace(win.astype(np.complex128), d, L, Y, Nb,
+                            pixels=(np.arange(len(ix)), np.zeros(len(ix), int)))
+    P = out["P"].cpu().numpy().reshape(Nx, Nz - 2 * Y)[ix, jj]
+    e = np.abs(P - ref["P"]) / ref["P"]
+    gbs = bpx * npx / (ms * 1e-3) / 1e9
+    line = _base(args, "Capon pixels/sec, sonar beamspace M=96 L=48 Nb=5 (sliding DFT)", npx / (ms * 1e-3),
+                 "pixels/s", ms, {"workload": "sonar beamspace frame", "Nx": Nx, "Nz": Nz, "M": M, "L": L,
+                                  "Nb": Nb, "Yavg": Y, "pixels": npx})
+    line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_pixel": bpx,
+                        "fp32_tflops": round(fpx * npx / (ms * 1e-3) / 1e12, 2), "flop_per_pixel": fpx}
+    line["accuracy"] = {"P_rel_err_max_128px": float(e.max())}
+    return line
+
+
+def steering(args):
+    import torch
+    from cos_amd import CaponPlan, synth
+    from oracle import capon_oracle as O
+    L, Y, d = 16, 2, 0.01
+    xn = synth.config1(N=1024, M=32, seed=1)
+    sines = synth.steering_sines(128)
+    x = torch.from_numpy(xn).cuda()
+    st = torch.from_numpy(sines.astype(np.float32)).cuda()
+    plan = CaponPlan()
+    step = lambda: plan.run(x, d, L, Y, steer_sin=st, want=("P", "w"))
+    for _ in range(args.warmup):
+        o = step()
+    ms = _timed(torch, step, args.steps)
+    o = step()
+    torch.cuda.synchronize()
+    ref = O.capon_steered(xn.astype(np.complex128), d, L, Y, sines.astype(np.float32).astype(np.float64))
+    P = o["P"].cpu().numpy().reshape(-1, 128)
+    e = np.abs(P - ref["P"]) / ref["P"]
+    npow = P.size
+    line = _base(args, "Capon steered powers/sec, ULA M=32 L=16, 128 angles per range bin", npow / (ms * 1e-3),
+                 "powers/s", ms, {"workload": "far-field steered ULA", "bins": 1020, "angles": 128, "M": 32, "L": L,
+                                  "Yavg": Y, "factorisations": 1020})
+    line["accuracy"] = {"P_rel_err_max": float(e.max())}
+    return line
+
+
+def adjoint(args):
+    import torch
+    from cos_amd import synth
+    from cos_amd.autograd import capon_power
+    F, Nx, Nz, M, L, Y, d = 8, 128, 512, 64, 32, 2, 0.01
+    frames = torch.stack([synth.speckle_cube_torch(Nx, Nz, M, seed=50 + f, device="cuda") for f in range(F)])
+    phi = torch.from_numpy(synth.config5_phase(M, seed=5)).cuda().requires_grad_(True)
+    npx = F * Nx * (Nz - 2 * Y)
+
+    def fwd():
+        xs = (frames * torch.exp(1j * phi)).reshape(F * Nx, Nz, M)
+        return -torch.log(capon_power(xs, d, L, Y)).mean()
+
+    def step():
+        phi.grad = None
+        fwd().backward()
+
+    for _ in range(args.warmup):
+        step()
+    ms = _timed(torch, step, args.steps)
+    with torch.no_grad():
+        ms_f = _timed(torch, lambda: fwd(), args.steps)
+    line = _base(args, "Capon pixels/sec through forward + hand-written adjoint (dLoss/dphi, dLoss/dx)",
+                 npx / (ms * 1e-3), "pixels/s", ms,
+                 {"workload": "differentiable frames", "frames": F, "Nx": Nx, "Nz": Nz, "M": M, "L": L,
+                  "Yavg": Y, "pixels": npx})
+    line["forward_ms"] = round(ms_f, 3)
+    line["backward_over_forward"] = round((ms - ms_f) / ms_f, 2)
+    return line

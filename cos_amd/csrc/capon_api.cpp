@@ -53,6 +53,15 @@ struct DevBuf {
 
 }  // namespace
 
+// Output rows jlo .. jlo+jn-1 whose centre-row L falls in one lane-shape bucket.
+struct RowRun {
+  int jlo, jn, Lmax;
+};
+
+// Lane-shape buckets of the fused kernel: 16-lane groups with one or two columns per
+// lane, or a whole wave per pixel.
+inline int bucket_cap(int L) { return L <= 16 ? 16 : L <= 32 ? 32 : 64; }
+
 // The frame's launches captured once and replayed: valid while the descriptor, every
 // pointer and wstride are unchanged.
 struct FrameGraph {
@@ -79,6 +88,7 @@ struct cos_capon_plan {
   hipStream_t last_stream = nullptr;
   hipStream_t capture_stream = nullptr;  // graphs are captured here, launched on the caller's
   FrameGraph fg;
+  std::vector<RowRun> runs;   // L-bucket runs of output rows for table-driven frames
   bool fallback_ran = false;
   ~cos_capon_plan() {
     fg.reset();
@@ -128,6 +138,8 @@ cosk::CaponLaunch base_launch(const cos_capon_desc& ds, const void* x) {
   a.rowOff = ds.row_off;
   a.rowD = ds.row_d;
   a.npix = ds.Nx * (ds.Nz - 2 * ds.Yavg);
+  a.jlo = 0;
+  a.jn = ds.Nz - 2 * ds.Yavg;
   const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
   a.Lcap = tables ? ds.Lmax : ds.L;
   a.Mcap = tables ? ds.Mmax : ds.M;
@@ -196,7 +208,6 @@ int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
   if (ds.nsteer < 0 || (ds.nsteer > 0 && !ds.steer_sin)) return fail(hipErrorInvalidValue, "bad steering");
   if (ds.precision == COS_PREC_FP32_FALLBACK && ds.nsteer > 0)
     return fail(hipErrorInvalidValue, "FP64 fallback does not support steering");
-  if (ds.precision == COS_PREC_FP32_FALLBACK && Lmax > 32) return fail(hipErrorInvalidValue, "FP64 fallback supports L <= 32");
   return 0;
 }
 
@@ -236,8 +247,20 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
   uint8_t* fl = (uint8_t*)flags;
   if (fb && !fl) fl = plan->flags.as<uint8_t>();
   a.flags = fl;
-  e = cosk::launch_capon_f32(a, Lmax, Mmax, s, &why);
-  if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
+  if (tables && !plan->runs.empty()) {
+    // range rows grouped into runs of one L bucket, each launched with its lane shape
+    for (const RowRun& r : plan->runs) {
+      cosk::CaponLaunch b = a;
+      b.jlo = r.jlo;
+      b.jn = r.jn;
+      b.npix = ds.Nx * r.jn;
+      e = cosk::launch_capon_f32(b, r.Lmax, Mmax, s, &why);
+      if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
+    }
+  } else {
+    e = cosk::launch_capon_f32(a, Lmax, Mmax, s, &why);
+    if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
+  }
   if (!fb) return 0;
   e = cosk::compact_flags(fl, a.npix, plan->list.as<int>(), plan->count.as<int>(), s);
   if (e != hipSuccess) return fail_hip(e, "compaction");
@@ -274,11 +297,33 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   }
   plan->last_stream = s;
   plan->fallback_ran = fb;
-  if (!ds.graph) return enqueue_frame(plan, ds, x, P, z, w, wstride, R, flags, s, fb);
-
   const void* ptr[7] = {x, P, z, w, R, flags, nullptr};
   FrameGraph& g = plan->fg;
-  if (!(g.valid && same_desc(g.ds, ds) && std::memcmp(g.ptr, ptr, sizeof(ptr)) == 0 && g.wstride == wstride)) {
+  const bool replay = ds.graph && g.valid && same_desc(g.ds, ds) && std::memcmp(g.ptr, ptr, sizeof(ptr)) == 0 &&
+                      g.wstride == wstride;
+  plan->runs.clear();
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  if (tables && !replay) {
+    // Read the L table (Nz ints) to split the rows into L-bucket runs; more than 16 runs
+    // fall back to a single launch sized for Lmax.
+    // (ordered after the caller's work on s, which may still be writing the table)
+    std::vector<int> hL(ds.Nz);
+    if ((e = hipMemcpyAsync(hL.data(), ds.row_L, sizeof(int) * ds.Nz, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+      return fail_hip(e, "read L table");
+    const int Nzp = ds.Nz - 2 * ds.Yavg;
+    for (int jj = 0; jj < Nzp;) {
+      const int cap = bucket_cap(hL[jj + ds.Yavg]);
+      int j2 = jj + 1;
+      while (j2 < Nzp && bucket_cap(hL[j2 + ds.Yavg]) == cap) ++j2;
+      plan->runs.push_back({jj, j2 - jj, cap < ds.Lmax ? cap : ds.Lmax});
+      jj = j2;
+    }
+    if (plan->runs.size() > 16) plan->runs.clear();
+  }
+  if (!ds.graph) return enqueue_frame(plan, ds, x, P, z, w, wstride, R, flags, s, fb);
+
+  if (!replay) {
     g.reset();
     if (!plan->capture_stream && (e = hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking)) != hipSuccess)
       return fail_hip(e, "capture stream");

@@ -79,7 +79,7 @@ __device__ __noinline__ cplx<T> steered_mean(const float2* xc, int c, int K, int
 
 // Occupancy target (waves per SIMD) the register allocator is held to.
 template <typename T, int G, int CPL> struct OccHint {
-  static constexpr int value = sizeof(T) == 4 ? (G == 16 ? (CPL == 1 ? 4 : 2) : 1) : (CPL == 1 ? 2 : 1);
+  static constexpr int value = sizeof(T) == 4 ? (G == 16 ? (CPL == 1 ? 4 : 2) : 1) : (G == 16 && CPL == 1 ? 2 : 1);
 };
 
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool BWD = false>
@@ -95,7 +95,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   const int gid = gid0 + q;
   const bool active = gid < npix;
   const int gsel = active ? gid : gid0;
-  const int p = a.plist ? a.plist[gsel] : gsel;
+  const int p = a.plist ? a.plist[gsel] : (gsel / a.jn) * Nzp + a.jlo + gsel % a.jn;
   const int ix = p / Nzp, j = p - ix * Nzp, crow = j + Y;
   int L = a.rowL ? a.rowL[crow] : a.L;
   int M = a.rowM ? a.rowM[crow] : a.M;

@@ -21,6 +21,7 @@ struct CaponLaunch {
   const int* plist;    // optional pixel list (values p); null = all pixels
   const int* pcount;   // optional device count for plist (overrides npix when set)
   int npix;            // number of pixels (list capacity when pcount is set)
+  int jlo, jn;         // without plist: output rows jlo .. jlo+jn-1 of every beam (npix = Nx*jn)
   float* P;            // [npix_total] power 1/(a^H R_L^-1 a), R normalised by K(2Y+1)
   float2* z;           // [npix_total] amplitude w^H g
   float2* w;           // [npix_total][wstride] weights (first L entries)

@@ -191,14 +191,15 @@ def _t_mix(z):
     return z ^ ((z >> 31) & m31)
 
 
-def cnormal_torch(seed: int, stream: int, shape, device, chunk: int = 1 << 24):
+def cnormal_torch(seed: int, stream: int, shape, device, chunk: int = 1 << 24, dtype=None):
     """Same numbers as cnormal() (complex128 arithmetic, returned as complex64) computed
     with torch on `device`, in chunks."""
     import torch
     n = int(np.prod(shape))
     key = int(np.array([_key(seed, stream)], dtype=np.uint64).view(np.int64)[0])
     gold = int(np.array([GOLDEN], dtype=np.uint64).view(np.int64)[0])
-    out = torch.empty(n, dtype=torch.complex64, device=device)
+    dtype = dtype or torch.complex64
+    out = torch.empty(n, dtype=dtype, device=device)
     m53 = (1 << 53) - 1
     for s in range(0, n, chunk):
         e = min(n, s + chunk)
@@ -207,7 +208,7 @@ def cnormal_torch(seed: int, stream: int, shape, device, chunk: int = 1 << 24):
         u = ((z >> 11) & m53).to(torch.float64) * (1.0 / 9007199254740992.0)
         r = torch.sqrt(-torch.log1p(-u[0::2]))
         th = 2.0 * np.pi * u[1::2]
-        out[s:e] = torch.complex(r * torch.cos(th), r * torch.sin(th)).to(torch.complex64)
+        out[s:e] = torch.complex(r * torch.cos(th), r * torch.sin(th)).to(dtype)
         del ctr, z, u, r, th
     return out.reshape(shape)
 
@@ -230,3 +231,40 @@ def config4_torch(Nx: int = 512, Nz: int = 8192, M: int = 96, seed: int = 4, dev
         ph = torch.exp(1j * np.pi * m[None, :] * (np.sin(th) - looks[:, None]))       # [Nx][M]
         x[:, z0:z0 + span, :] += (sig[None, :, None] * ph[:, None, :]).to(torch.complex64)
     return x
+
+
+def speckle_cube_torch(Nx: int, Nz: int, M: int, seed: int, device="cuda", n_scatterers: int = 10,
+                       scatterer_db: float = 30.0, stream: int = 0):
+    """speckle_cube() computed with torch on `device` (same recipe, same numbers up to the
+    final complex64 rounding)."""
+    import torch
+    n = cnormal_torch(seed, stream, (Nx, Nz + 4, M), device, dtype=torch.complex128)
+    h = AXIAL_PULSE
+    cube = torch.zeros((Nx, Nz, M), dtype=torch.complex128, device=device)
+    for i in range(5):
+        cube += float(h[i]) * n[:, i:i + Nz, :]
+    del n
+    var = float(np.sum(h * h))
+    u = uniforms(seed, stream + 1, 3 * n_scatterers)
+    amp = np.sqrt(10.0 ** (scatterer_db / 10.0) * var)
+    for s in range(n_scatterers):
+        zs = min(int(u[3 * s] * Nz), Nz - 1)
+        xs = min(int(u[3 * s + 1] * Nx), Nx - 1)
+        a = amp * np.exp(2j * np.pi * u[3 * s + 2])
+        for i in range(5):
+            zz = zs + i - 2
+            if 0 <= zz < Nz:
+                cube[xs, zz, :] += complex(h[i] * a)
+    return cube.to(torch.complex64)
+
+
+def config3_torch(Nx: int = 256, Nz: int = 4096, seed: int = 3, Mmax: int = 128, device="cuda"):
+    """config3() on a torch device: (cube tensor, M_rows, L_rows, off_rows, d_rows numpy)."""
+    import torch
+    Mz, Lz, off, dz = config3_geometry(Nz, seed, Mmax=Mmax)
+    cube = speckle_cube_torch(Nx, Nz, Mmax, seed, device=device)
+    m = torch.arange(Mmax, device=device)
+    o = torch.from_numpy(off).to(device)[:, None]
+    mz = torch.from_numpy(Mz).to(device)[:, None]
+    cube *= ((m[None, :] >= o) & (m[None, :] < o + mz))[None].to(cube.dtype)
+    return cube, Mz, Lz, off, dz

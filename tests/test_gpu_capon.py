@@ -144,6 +144,42 @@ def test_rows_tables_and_fp64_fallback():
         np.testing.assert_allclose(o["w"][n, :Ln], ref["w"][n], atol=2e-3 * np.abs(ref["w"][n]).max())
 
 
+def test_expanding_aperture_full_depth():
+    """The expanding-aperture geometry over its full depth (M 16..128, L 8..64, 5% of rows
+    at d = 1e-5) on two beams: the frame is split into runs by subarray-length bucket,
+    flagged pixels with L up to 64 go through the FP64 pass.  Every 5th pixel against the
+    FP64 oracle; power within 1e-4, FP64-recomputed pixels within float32 rounding."""
+    import torch
+    from cos_amd import synth
+    say("expanding aperture full depth", 2, 4096, 128)
+    Nx, Nz, Y = 2, 4096, 2
+    Mz, Lz, off, dz = synth.config3_geometry(Nz, seed=3, Mmax=128)
+    x = synth.speckle_cube(Nx, Nz, 128, seed=3)
+    m = np.arange(128)
+    x *= ((m[None, :] >= off[:, None]) & (m[None, :] < (off + Mz)[:, None]))[None]
+    rows = {k: torch.from_numpy(v).cuda() for k, v in dict(L=Lz, M=Mz, off=off, d=dz).items()}
+    o, plan = _plan_run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", want=("P", "w", "flags"))
+    Nzp = Nz - 2 * Y
+    sel = np.arange(0, Nx * Nzp, 5)
+    ref = O.capon_rows(x.astype(np.complex128), dz.astype(np.float64), Lz, Mz, off, Y,
+                       pixels=(sel // Nzp, sel % Nzp))
+    e = rel(o["P"][sel], ref["P"])
+    print(f"aperture P rel err max {e.max():.3e}; fallback {plan.fallback_count()}", flush=True)
+    assert e.max() <= 1e-4
+    c = np.tile(np.arange(Nzp) + Y, Nx)
+    ill = (1 + Lz[c] / dz[c].astype(np.float64)) > 1e6
+    flagged = np.nonzero(o["flags"])[0]
+    assert set(np.nonzero(ill)[0].tolist()) <= set(flagged.tolist())
+    assert (Lz[c[flagged]] > 32).any(), "the FP64 pass must see subarrays longer than 32"
+    fsel = np.isin(sel, flagged)
+    assert fsel.any()
+    assert rel(o["P"][sel[fsel]], ref["P"][fsel]).max() <= 1.2e-7
+    for n in range(0, len(sel), 97):
+        Ln = len(ref["w"][n])
+        np.testing.assert_allclose(o["w"][sel[n], :Ln], ref["w"][n], atol=2e-3 * np.abs(ref["w"][n]).max())
+        assert np.all(o["w"][sel[n], Ln:] == 0)
+
+
 # ------------------------------------------------------------------ steering vectors
 
 def test_steering_matches_oracle():

@@ -31,3 +31,12 @@ def test_speckle_variance_and_scatterers():
     y = synth.speckle_cube(16, 64, 8, seed=9, n_scatterers=3)
     d = np.abs(y - x).max(axis=2)
     assert (d > 10).sum() >= 3                  # scatterer cells stand out, coherent over M
+
+
+def test_torch_speckle_matches_numpy():
+    a = synth.speckle_cube(4, 30, 8, seed=3)
+    b = synth.speckle_cube_torch(4, 30, 8, seed=3, device="cpu").numpy()
+    assert np.array_equal(a, b)
+    c, *_ = synth.config3(Nx=2, Nz=64, seed=3, Mmax=32)
+    d, *_ = synth.config3_torch(Nx=2, Nz=64, seed=3, Mmax=32, device="cpu")
+    assert np.array_equal(c, d.numpy())
