@@ -239,6 +239,17 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
   const char* why = nullptr;
   hipError_t e;
   if (ds.precision == COS_PREC_FP64) {
+    if (tables && !plan->runs.empty()) {
+      for (const RowRun& r : plan->runs) {
+        cosk::CaponLaunch b = a;
+        b.jlo = r.jlo;
+        b.jn = r.jn;
+        b.npix = ds.Nx * r.jn;
+        e = cosk::launch_capon_f64(b, r.Lmax, Mmax, s, &why);
+        if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 kernel");
+      }
+      return 0;
+    }
     e = cosk::launch_capon_f64(a, Lmax, Mmax, s, &why);
     if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 kernel");
     return 0;

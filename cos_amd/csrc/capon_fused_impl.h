@@ -26,12 +26,12 @@
 // The FP64 path ("plain" per-pixel kernel, also the fallback for ill-conditioned
 // pixels) uses direct sums for every element.
 //
-// Cholesky: right-looking U^H U on the upper triangle; row k is broadcast lane by
-// lane with DPP row_newbcast (G=16) or v_readlane (G=64).  The forward solves with
-// U^H for the two right-hand sides a and g are column-oriented and local; their
-// running dot products give a^H R^-1 a = |U^-H a|^2 and y^H g = (U^-H a)^H (U^-H g)
-// without a reduction.  The back solve U y = U^-H a (only needed for the weights)
-// is row-oriented and uses one group reduction per row.
+// Cholesky: right-looking U^H U on the upper triangle.  Each finished row k of U is
+// published in LDS and read back by the whole group (broadcast reads), so the trailing
+// update is FMAs only; the same row is stored packed for the back solve, and the
+// forward solve U^H v = a runs inside the factorisation (a as an extra column), so the
+// registers of A die row by row.  a^H S_L^-1 a = |v|^2.  The back solve U y = v is
+// column-oriented: one broadcast of y_j per column, U[i][j] read from LDS.
 #pragma once
 #include "cos_device.h"
 #include "capon_kernels.h"
@@ -356,118 +356,162 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     gS[S] = {acc.x * invK, acc.y * invK};
   });
 
-  // ---------------- Cholesky U^H U = S_L ----------------
-  bool notpd = false;
+  // ---------------- Cholesky U^H U = S_L, fused forward solve U^H v = a ----------------
+  // LDS region of the group (free now: every read of the snapshots is complete), in
+  // cplx<T> units:
+  //   Upk [0, NU)           strictly upper U[r][c] (r < c) at r + c(c+1)/2; the diagonal
+  //                         slot c + c(c+1)/2 holds v_c, then 0 for the back solve
+  //   dump NU               target of the row-k writes of lanes whose column is <= k
+  //   rowb [NU+2, NU+2+2L)  row k of U, published for the trailing update (broadcast
+  //                         reads); two buffers alternate so row k+1 can be published
+  //                         while row k's updates still read the other one
+  //   invd (T) at the end   1/U_kk
+  // Row k is written to Upk as it is finished, so the registers of A die row by row.
+  constexpr int NU = LMAX * (LMAX + 1) / 2;
+  cplx<T>* const Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
+  cplx<T>* const rowb = Upk + NU + 2;
+  T* const invd = reinterpret_cast<T*>(xs_all + (q + 1) * GREG) - LMAX;
+  int ucol[CPL];   // Upk offset of this lane's columns
+  static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+    ucol[S] = cs[S] * (cs[S] + 1) / 2;
+  });
+  cplx<T> b1[CPL];   // right-hand side a = 1 (0 in the padding), reduced column by column
+  static_for<0, CPL>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+    b1[S] = {cs[S] < L ? T(1) : T(0), T(0)};
+  });
   static_for<0, LMAX>([&](auto Kk_) {
     constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
-    T piv = Gp::template bcast<lk>(A[slot<G, sk, k>()].x) + ((k < L) ? delta : T(0));
-    const bool ok = piv > tiny_value<T>();
-    notpd |= !ok;
-    piv = ok ? piv : tiny_value<T>();
+    // A pivot that is not positive (or NaN) is clamped to tiny; the flag pass detects
+    // the clamp from the largest 1/U_kk.  The loading is added in the identity padding
+    // too, where it changes nothing (its right-hand side is 0).
+    const T piv = fmax(Gp::template bcast<lk>(A[slot<G, sk, k>()].x) + delta, tiny_value<T>());
     const T inv = rsqrt_t<T>(piv);
+    invd[k] = inv;
+    cplx<T>* const rb = rowb + (k & 1) * LMAX;
     static_for<0, CPL>([&](auto S_) {
       constexpr int S = decltype(S_)::value;
       if constexpr (k < G * (S + 1)) {
         A[slot<G, S, k>()].x *= inv;
         A[slot<G, S, k>()].y *= inv;
+        rb[cs[S]] = A[slot<G, S, k>()];
+        Upk[k < cs[S] ? k + ucol[S] : NU] = A[slot<G, S, k>()];
       }
     });
-    if (g == lk) A[slot<G, sk, k>()] = {piv * inv, inv};
+    // forward step: v_k = b_k / U_kk, b_c -= conj(U[k][c]) v_k
+    cplx<T> v1 = bcastc<G, lk>(b1[sk]);
+    v1.x *= inv;
+    v1.y *= inv;
+    Upk[k + k * (k + 1) / 2] = v1;
+    static_for<0, CPL>([&](auto S_) {
+      constexpr int S = decltype(S_)::value;
+      if constexpr (k < G * (S + 1)) sub_conj_mul(b1[S], A[slot<G, S, k>()], v1);
+    });
+    // trailing update A[i][c] -= conj(U[k][i]) U[k][c]
     static_for<k + 1, LMAX>([&](auto Ii_) {
-      constexpr int i = decltype(Ii_)::value, si = i / G, li = i % G;
-      const cplx<T> u = bcastc<G, li>(A[slot<G, si, k>()]);
+      constexpr int i = decltype(Ii_)::value;
+      const cplx<T> u = rb[i];
       static_for<0, CPL>([&](auto S_) {
         constexpr int S = decltype(S_)::value;
         if constexpr (i < G * (S + 1)) sub_conj_mul(A[slot<G, S, i>()], u, A[slot<G, S, k>()]);
       });
     });
   });
-
-  // Strictly upper U[r][c] (r < c) packed at r + c(c+1)/2 into this group's LDS region,
-  // which held the snapshots (all reads of them are complete: the values above depend
-  // on them).  The back solve reads it row-wise.
-  const bool want_y = a.w || a.z || BWD;
-  cplx<T>* Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
-  if (want_y) {
-    static_for<0, CPL>([&](auto S_) {
-      constexpr int S = decltype(S_)::value;
-      static_for<0, G * (S + 1)>([&](auto R_) {
-        constexpr int R = decltype(R_)::value;
-        const int cc = cs[S];
-        if (R < cc) Upk[R + cc * (cc + 1) / 2] = A[slot<G, S, R>()];
-      });
+  // a clamped pivot gives 1/U_kk = rsqrt(tiny): the factorisation broke down
+  bool notpd = false;
+  if (a.flags) {
+    T maxinv = T(0);
+    static_for<0, LMAX>([&](auto Kk_) {
+      constexpr int k = decltype(Kk_)::value;
+      maxinv = fmax(maxinv, invd[k]);
     });
+    notpd = !(maxinv < T(0.5) * rsqrt_t<T>(tiny_value<T>()));
   }
 
+  const bool want_y = a.w || a.z || BWD;
   const T norm = (T)K * (T)TW;
   const int nrhs = steered ? a.nsteer : 1;
   const float2* xcg = a.x + ((size_t)ix * a.Nz + (j + Y)) * a.Mstride + off;   // centre row (steering)
 
   for (int ia = 0; ia < nrhs; ++ia) {
     const T sn = steered ? (T)a.steer_sin[ia] : T(0);
-    cplx<T> b1[CPL], zA[CPL], gcur[CPL];
+    cplx<T> gcur[CPL];
     static_for<0, CPL>([&](auto S_) {
       constexpr int S = decltype(S_)::value;
-      const int cc = cs[S];
-      zA[S] = {T(0), T(0)};
-      b1[S] = {T(1), T(0)};
       gcur[S] = gS[S];
-      if (steered) {
-        float sv, cv;
-        sincospif((float)(sn * (T)cc), &sv, &cv);
-        b1[S] = {(T)cv, (T)sv};
-        gcur[S] = steered_mean<T>(xcg, cc, K, L, sn, invK);
-      }
-      if (cc >= L) b1[S] = {T(0), T(0)};
     });
-
-    // forward solve U^H v = a; |v|^2 = a^H S_L^-1 a
-    T accP = T(0);
-    static_for<0, LMAX>([&](auto Kk_) {
-      constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
-      const T inv = Gp::template bcast<lk>(A[slot<G, sk, k>()].y);
-      cplx<T> v1 = bcastc<G, lk>(b1[sk]);
-      v1.x *= inv;
-      v1.y *= inv;
-      accP = fma(v1.x, v1.x, fma(v1.y, v1.y, accP));
-      if (g == lk) zA[sk] = v1;
+    if (steered) {
+      // one factorisation, many right-hand sides: forward solve for a_theta with the
+      // rows of U read back from Upk
+      cplx<T> bs[CPL];
       static_for<0, CPL>([&](auto S_) {
         constexpr int S = decltype(S_)::value;
-        if constexpr (k < G * (S + 1)) sub_conj_mul(b1[S], A[slot<G, S, k>()], v1);
+        const int cc = cs[S];
+        float sv, cv;
+        sincospif((float)(sn * (T)cc), &sv, &cv);
+        bs[S] = cc < L ? cplx<T>{(T)cv, (T)sv} : cplx<T>{T(0), T(0)};
+        gcur[S] = steered_mean<T>(xcg, cc, K, L, sn, invK);
       });
+      static_for<0, LMAX>([&](auto Kk_) {
+        constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
+        cplx<T> v1 = bcastc<G, lk>(bs[sk]);
+        const T inv = invd[k];
+        v1.x *= inv;
+        v1.y *= inv;
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          if constexpr (k < G * (S + 1)) {
+            const cplx<T> u = Upk[k < cs[S] ? k + ucol[S] : NU];
+            sub_conj_mul(bs[S], u, v1);
+          }
+        });
+        Upk[k + k * (k + 1) / 2] = v1;
+      });
+    }
+
+    // v (the forward solution, one entry per column) sits in the diagonal slots;
+    // |v|^2 = a^H S_L^-1 a is a group sum over the lanes' columns
+    cplx<T> cr[CPL];
+    T accP = T(0);
+    static_for<0, CPL>([&](auto S_) {
+      constexpr int S = decltype(S_)::value;
+      cr[S] = Upk[cs[S] + ucol[S]];
+      accP = fma(cr[S].x, cr[S].x, fma(cr[S].y, cr[S].y, accP));
     });
+    accP = Gp::sum(accP);
 
     const size_t po = (size_t)p * nrhs + ia;
     const float nanv = __builtin_nanf("");
     if (active && g == 0 && a.P) a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
 
     if (want_y) {
-      // back solve U y = v with the rows of U read from LDS; column-oriented:
-      // y_j = c_j / U_jj, then c_i -= U[i][j] y_j for the rows i < j this lane owns.
-      cplx<T> y[CPL], cr[CPL];
+      // back solve U y = v, column-oriented: y_j = c_j / U_jj, then c_i -= U[i][j] y_j
+      // for the rows i < j this lane owns (U[i][j] read from Upk; lanes whose row is
+      // >= j read the zeroed diagonal slot of column j and leave c_i unchanged).
       static_for<0, CPL>([&](auto S_) {
         constexpr int S = decltype(S_)::value;
-        y[S] = {T(0), T(0)};
-        cr[S] = zA[S];
+        Upk[cs[S] + ucol[S]] = cplx<T>{T(0), T(0)};
       });
-      static_for<LMAX - 1, -1, -1>([&](auto Cc_) {
+      static_for<LMAX - 1, 0, -1>([&](auto Cc_) {
         constexpr int jj = decltype(Cc_)::value, sj = jj / G, lj = jj % G;
-        const T inv = A[slot<G, sj, jj>()].y;                 // owner lane: 1/U_jj
-        const cplx<T> val = {cr[sj].x * inv, cr[sj].y * inv};
-        if (g == lj) y[sj] = val;
-        if constexpr (jj > 0) {
-          const cplx<T> yb = bcastc<G, lj>(val);
-          static_for<0, CPL>([&](auto S_) {
-            constexpr int S = decltype(S_)::value;
-            if constexpr (G * S < jj) {                      // some row i = g + G*S < jj
-              const int ii = min(cs[S], jj);
-              const cplx<T> u = Upk[ii + jj * (jj + 1) / 2];
-              cplx<T> t = cr[S];
-              add_mul(t, cplx<T>{-u.x, -u.y}, yb);
-              if (cs[S] < jj) cr[S] = t;
-            }
-          });
-        }
+        const T inv = invd[jj];
+        cplx<T> yb = bcastc<G, lj>(cr[sj]);
+        yb.x *= inv;
+        yb.y *= inv;
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          if constexpr (G * S < jj) {                      // some row i = g + G*S < jj
+            const cplx<T> u = Upk[min(cs[S], jj) + jj * (jj + 1) / 2];
+            sub_mul(cr[S], u, yb);
+          }
+        });
+      });
+      cplx<T> y[CPL];
+      static_for<0, CPL>([&](auto S_) {
+        constexpr int S = decltype(S_)::value;
+        const T inv = invd[cs[S]];
+        y[S] = {cr[S].x * inv, cr[S].y * inv};
       });
       // w = y / (a^H y), a^H y = |v|^2;  z = w^H g
       const T sc = T(1) / accP;
@@ -609,8 +653,10 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   constexpr int NG = 64 / G, LMAX = G * CPL;
   const int MP = ((Mmax + LMAX + 4 + 3) / 4) * 4;   // zero padding covers every lag read
   const int TW = 2 * a.Y + 1;
-  // per-group LDS region: the window's snapshots, later the packed upper U
-  const int upk = LMAX * (LMAX + 1) / 2 * (int)(sizeof(cplx<T>) / sizeof(float2));
+  // per-group LDS region: the window's snapshots, later the packed upper U + dump + pad,
+  // two row buffers and 1/U_kk (layout in capon_group)
+  const int upk = (LMAX * (LMAX + 1) / 2 + 2 + 2 * LMAX) * (int)(sizeof(cplx<T>) / sizeof(float2)) +
+                  LMAX * (int)sizeof(T) / (int)sizeof(float2);
   int GREG = TW * MP > upk ? TW * MP : upk;
   if (BWD) {   // window rows plus beta[K <= BWD_KMAX] for the adjoint epilogue
     if (Mmax > BWD_KMAX) return hipErrorInvalidValue;

@@ -130,6 +130,14 @@ __device__ __forceinline__ void add_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
   a.y = fma(u.x, v.y, a.y);
   a.y = fma(u.y, v.x, a.y);
 }
+// a -= u * v
+template <typename T>
+__device__ __forceinline__ void sub_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
+  a.x = fma(-u.x, v.x, a.x);
+  a.x = fma(u.y, v.y, a.x);
+  a.y = fma(-u.x, v.y, a.y);
+  a.y = fma(-u.y, v.x, a.y);
+}
 // a += conj(u) * v
 template <typename T>
 __device__ __forceinline__ void add_conj_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
