@@ -47,6 +47,10 @@ __device__ __forceinline__ cplx<T> ldx(const float2* p) {
 // Largest K = M - L + 1 the adjoint epilogue handles (its per-lane k and m slots).
 constexpr int BWD_KMAX = 128;
 
+// Rows of each Cholesky step whose multiplier U[k][i] is broadcast from registers
+// rather than read back from LDS.
+constexpr int CHOL_REG_ROWS = 10;
+
 template <int G, int S, int R>
 __device__ __forceinline__ constexpr int slot() { return G * S * (S + 1) / 2 + R; }
 
@@ -78,8 +82,11 @@ __device__ __noinline__ cplx<T> steered_mean(const float2* xc, int c, int K, int
 }
 
 // Occupancy target (waves per SIMD) the register allocator is held to.
-template <typename T, int G, int CPL> struct OccHint {
-  static constexpr int value = sizeof(T) == 4 ? (G == 16 ? (CPL == 1 ? 4 : 2) : 1) : (G == 16 && CPL == 1 ? 2 : 1);
+// (16-lane groups with one column per lane: 4, or 3 when a window of more than five
+// rows keeps its alpha/beta columns in registers)
+template <typename T, int G, int CPL, int TM> struct OccHint {
+  static constexpr int value = sizeof(T) == 4 ? (G == 16 ? (CPL == 1 ? (TM == 0 || TM > 5 ? 3 : 4) : 2) : 1)
+                                              : (G == 16 && CPL == 1 ? 2 : 1);
 };
 
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool BWD = false>
@@ -274,9 +281,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         });
       };
       if constexpr (TM > 0) {
+        // branch-free over the window bound: rows past the window repeat the last row
+        // against zero alpha/beta, so every LDS read of the row can be issued together
         static_for<0, TM>([&](auto t_) {
-    constexpr int t = decltype(t_)::value;
-          if (t < TW) corr(t, al[t], be[t]);
+          constexpr int t = decltype(t_)::value;
+          corr(t < TW ? t : TW - 1, al[t], be[t]);
         });
       } else {
         for (int t = 0; t < TW; ++t) {
@@ -368,6 +377,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   //   invd (T) at the end   1/U_kk
   // Row k is written to Upk as it is finished, so the registers of A die row by row.
   constexpr int NU = LMAX * (LMAX + 1) / 2;
+  // FP32 16-lane groups broadcast U[k][i] inside the FMAs (v_fmac_f32 with a DPP source);
+  // other shapes publish row k in LDS
+  constexpr bool DPP_FMA = std::is_same<T, float>::value && G == 16;
   cplx<T>* const Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
   cplx<T>* const rowb = Upk + NU + 2;
   T* const invd = reinterpret_cast<T*>(xs_all + (q + 1) * GREG) - LMAX;
@@ -395,7 +407,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       if constexpr (k < G * (S + 1)) {
         A[slot<G, S, k>()].x *= inv;
         A[slot<G, S, k>()].y *= inv;
-        rb[cs[S]] = A[slot<G, S, k>()];
+        if constexpr (!DPP_FMA) rb[cs[S]] = A[slot<G, S, k>()];
         Upk[k < cs[S] ? k + ucol[S] : NU] = A[slot<G, S, k>()];
       }
     });
@@ -409,14 +421,33 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       if constexpr (k < G * (S + 1)) sub_conj_mul(b1[S], A[slot<G, S, k>()], v1);
     });
     // trailing update A[i][c] -= conj(U[k][i]) U[k][c]
-    static_for<k + 1, LMAX>([&](auto Ii_) {
-      constexpr int i = decltype(Ii_)::value;
-      const cplx<T> u = rb[i];
-      static_for<0, CPL>([&](auto S_) {
-        constexpr int S = decltype(S_)::value;
-        if constexpr (i < G * (S + 1)) sub_conj_mul(A[slot<G, S, i>()], u, A[slot<G, S, k>()]);
+    if constexpr (DPP_FMA) {
+      // U[k][i] broadcast inside the FMAs from its owner lane
+      static_for<k + 1, LMAX>([&](auto Ii_) {
+        constexpr int i = decltype(Ii_)::value, si = i / G, li = i % G;
+        if constexpr (CPL == 2 && i < G)
+          sub_conj_bcast_mul2<li, true>(A[slot<G, 0, i>()], A[slot<G, 1, i>()], A[slot<G, si, k>()],
+                                        A[slot<G, 0, k>()], A[slot<G, 1, k>()]);
+        else
+          sub_conj_bcast_mul<li, true>(A[slot<G, CPL - 1, i>()], A[slot<G, si, k>()], A[slot<G, CPL - 1, k>()]);
       });
-    });
+    } else {
+      // the first rows take U[k][i] by a register broadcast, which covers the latency of
+      // the LDS reads for the rest
+      cplx<T> ur[LMAX];
+      static_for<k + 1 + CHOL_REG_ROWS, LMAX>([&](auto Ii_) {
+        constexpr int i = decltype(Ii_)::value;
+        ur[i] = rb[i];
+      });
+      static_for<k + 1, LMAX>([&](auto Ii_) {
+        constexpr int i = decltype(Ii_)::value;
+        const cplx<T> u = (i <= k + CHOL_REG_ROWS) ? bcastc<G, i % G>(A[slot<G, i / G, k>()]) : ur[i];
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          if constexpr (i < G * (S + 1)) sub_conj_mul(A[slot<G, S, i>()], u, A[slot<G, S, k>()]);
+        });
+      });
+    }
   });
   // a clamped pivot gives 1/U_kk = rsqrt(tiny): the factorisation broke down
   bool notpd = false;
@@ -630,7 +661,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, bool BWD = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
-                          amdgpu_waves_per_eu(BWD ? 1 : OccHint<T, G, CPL>::value)))
+                          amdgpu_waves_per_eu(BWD ? 1 : OccHint<T, G, CPL, TM>::value)))
 void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   // PERSIST: grid-stride over groups of NG pixels, so a launch over a device-side pixel
   // list (the FP64 fallback) can use a bounded grid; otherwise one block per group (a

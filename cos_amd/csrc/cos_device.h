@@ -147,6 +147,47 @@ __device__ __forceinline__ void add_conj_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
   a.y = fma(-u.y, v.x, a.y);
 }
 
+// a -= conj(u_L) * v for one or two accumulators, where u_L is u in lane L of the lane's
+// 16-lane row: the broadcast happens inside the FMAs (v_fmac_f32 with a DPP row_newbcast
+// source operand) instead of a separate DPP move per component.  NOP adds the two wait
+// states a DPP read needs after a VALU write of its source register; without it the
+// caller guarantees that u was written at least two instructions earlier.
+#define COS_FMAC_BC(D, S, V, NEG) "v_fmac_f32_dpp " D ", " NEG S ", " V " row_newbcast:%[l] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+#define COS_BC1                                                                                               \
+  COS_FMAC_BC("%[ax]", "%[ux]", "%[vx]", "-") COS_FMAC_BC("%[ax]", "%[uy]", "%[vy]", "-")                      \
+  COS_FMAC_BC("%[ay]", "%[ux]", "%[vy]", "-") COS_FMAC_BC("%[ay]", "%[uy]", "%[vx]", "")
+#define COS_BC1_OPS                                                                                           \
+  : [ax] "+v"(a.x), [ay] "+v"(a.y) : [ux] "v"(u.x), [uy] "v"(u.y), [vx] "v"(v.x), [vy] "v"(v.y), [l] "n"(L)
+#define COS_BC2                                                                                               \
+  COS_FMAC_BC("%[ax]", "%[ux]", "%[vx]", "-") COS_FMAC_BC("%[bx]", "%[ux]", "%[wx]", "-")                      \
+  COS_FMAC_BC("%[ax]", "%[uy]", "%[vy]", "-") COS_FMAC_BC("%[bx]", "%[uy]", "%[wy]", "-")                      \
+  COS_FMAC_BC("%[ay]", "%[ux]", "%[vy]", "-") COS_FMAC_BC("%[by]", "%[ux]", "%[wy]", "-")                      \
+  COS_FMAC_BC("%[ay]", "%[uy]", "%[vx]", "") COS_FMAC_BC("%[by]", "%[uy]", "%[wx]", "")
+#define COS_BC2_OPS                                                                                           \
+  : [ax] "+v"(a.x), [ay] "+v"(a.y), [bx] "+v"(b.x), [by] "+v"(b.y)                                             \
+  : [ux] "v"(u.x), [uy] "v"(u.y), [vx] "v"(v.x), [vy] "v"(v.y), [wx] "v"(w.x), [wy] "v"(w.y), [l] "n"(L)
+template <int L, bool NOP>
+__device__ __forceinline__ void sub_conj_bcast_mul(cplx<float>& a, cplx<float> u, cplx<float> v) {
+  if constexpr (NOP)
+    asm volatile("s_nop 1\n\t" COS_BC1 COS_BC1_OPS);
+  else
+    asm volatile(COS_BC1 COS_BC1_OPS);
+}
+// the same for two accumulators sharing u: a -= conj(u_L) v, b -= conj(u_L) w
+template <int L, bool NOP>
+__device__ __forceinline__ void sub_conj_bcast_mul2(cplx<float>& a, cplx<float>& b, cplx<float> u, cplx<float> v,
+                                                    cplx<float> w) {
+  if constexpr (NOP)
+    asm volatile("s_nop 1\n\t" COS_BC2 COS_BC2_OPS);
+  else
+    asm volatile(COS_BC2 COS_BC2_OPS);
+}
+#undef COS_BC2_OPS
+#undef COS_BC2
+#undef COS_BC1_OPS
+#undef COS_BC1
+#undef COS_FMAC_BC
+
 template <typename T> __device__ __forceinline__ T rsqrt_t(T v);
 // v_rsq_f32 (1 ulp); the FP32 path's accuracy budget is set by the covariance sums.
 template <> __device__ __forceinline__ float rsqrt_t<float>(float v) { return __builtin_amdgcn_rsqf(v); }
