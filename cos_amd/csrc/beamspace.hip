@@ -13,12 +13,14 @@
 //              (RealTimeCapon/capon_kernel.cu:162-233); element-space w = B^H w_bs
 //              (RealTimeCapon/Capon.cpp:415).
 //
-// Work split.  A block owns one beam and a run of range rows.  Phase 1: one thread per
+// Work split.  A block owns one beam and a run of RB range rows.  Phase 1: one thread per
 // snapshot row runs the sliding DFT across its K subarrays and accumulates that row's
-// Nb x Nb Gram and its subarray mean; both go to LDS, so each snapshot is transformed once
-// and shared by the 2Y+1 pixels whose window contains it.  Phase 2: one thread per pixel
-// sums its window's Grams, factors the Nb x Nb system in registers and writes the outputs.
-// The cube is read once: the kernel is bounded by HBM and FP32 VALU together.
+// Nb x Nb Gram and its subarray mean.  The rows are staged through LDS in windows of W
+// elements (each wave loads its 64 rows with coalesced loads, then every thread walks its
+// own row in the window), so the cube is read once with full cache lines.  Both results
+// go to LDS, so each snapshot is transformed once and shared by the 2Y+1 pixels whose
+// window contains it.  Phase 2: one thread per pixel sums its window's Grams, factors the
+// Nb x Nb system in registers and writes the outputs.
 #include "cos_device.h"
 #include "beamspace.h"
 
@@ -27,34 +29,62 @@ namespace cosk {
 template <int NB>
 struct BsTri { static constexpr int N = NB * (NB + 1) / 2; };
 
+constexpr int BS_W = 8;            // window width (elements) of the staged rows
+constexpr int BS_WP = BS_W + 1;    // padded row stride in LDS (complex), spreads the banks
+
+// dst[r][i] = x[row r of this wave][m0 + i] for i < BS_W (0 outside the row or the frame)
+__device__ __forceinline__ void bs_stage(float2* dst, const float2* xw, int nrows, int Mstride, int M, int m0,
+                                         int lane) {
+#pragma unroll
+  for (int rr = 0; rr < 64 / (64 / BS_W); ++rr) {
+    const int r = rr * (64 / BS_W) + lane / BS_W, i = lane % BS_W, m = m0 + i;
+    float2 v = make_float2(0.f, 0.f);
+    if (r < nrows && m >= 0 && m < M) v = xw[(size_t)r * Mstride + m];
+    dst[r * BS_WP + i] = v;
+  }
+}
+
 template <int NB, int RB>
 __global__ __launch_bounds__(RB) void beamspace_capon_kernel(const BeamspaceLaunch a) {
   constexpr int NT = BsTri<NB>::N;
-  __shared__ float2 gram[RB][NT];
-  __shared__ float2 smean[RB][NB];
+  constexpr int NWAVE = RB / 64;
+  constexpr int TILE = NWAVE * 2 * 64 * BS_WP;       // float2: tail + head window per wave
+  constexpr int P2 = RB * (NT + NB);                  // float2: Grams + subarray means
+  __shared__ float2 smem[TILE > P2 ? TILE : P2];
+  float2 (*gram)[NT] = reinterpret_cast<float2 (*)[NT]>(smem);
+  float2 (*smean)[NB] = reinterpret_cast<float2 (*)[NB]>(smem + RB * NT);
   const int Y = a.Y, TW = 2 * Y + 1, Nzp = a.Nz - 2 * Y, TP = RB - 2 * Y;
   const int ix = blockIdx.y;
   const int t0 = blockIdx.x * TP;
   const int r = threadIdx.x;
-  const int L = a.L, K = a.M - a.L + 1;
+  const int L = a.L, K = a.M - a.L + 1, M = a.M;
   const float rsL = a.rsqrtL;
 
   // ---------------- phase 1: sliding DFT and per-snapshot Gram ----------------
+  float2 G[NT];
+  float2 sm[NB];
+#pragma unroll
+  for (int e = 0; e < NT; ++e) G[e] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) sm[b] = make_float2(0.f, 0.f);
   {
-    const int t = t0 + r;
-    float2 G[NT];
-    float2 sm[NB];
+    const int wv = r / 64, lane = r % 64;
+    const int tw0 = t0 + wv * 64;                        // first snapshot row of this wave
+    const int nrows = min(64, a.Nz - tw0);               // rows of this wave inside the frame
+    const float2* xw = a.x + ((size_t)ix * a.Nz + tw0) * a.Mstride + a.off;
+    float2* tail = smem + wv * 2 * 64 * BS_WP;
+    float2* head = tail + 64 * BS_WP;
+    const float2* mine_t = tail + lane * BS_WP;
+    const float2* mine_h = head + lane * BS_WP;
+    float2 S[NB];
 #pragma unroll
-    for (int e = 0; e < NT; ++e) G[e] = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int b = 0; b < NB; ++b) sm[b] = make_float2(0.f, 0.f);
-    if (t < a.Nz) {
-      const float2* x = a.x + ((size_t)ix * a.Nz + t) * a.Mstride + a.off;
-      float2 S[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) S[b] = make_float2(0.f, 0.f);
-      for (int l = 0; l < L; ++l) {
-        const float2 xv = x[l];
+    for (int b = 0; b < NB; ++b) S[b] = make_float2(0.f, 0.f);
+    // S_0[b] = sum_{l<L} x[l] tw[b][l]
+    for (int m0 = 0; m0 < L; m0 += BS_W) {
+      bs_stage(tail, xw, nrows, a.Mstride, M, m0, lane);
+      for (int i = 0; i < BS_W && m0 + i < L; ++i) {
+        const float2 xv = mine_t[i];
+        const int l = m0 + i;
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
           const float2 tw = a.tw[b * L + l];
@@ -62,41 +92,50 @@ __global__ __launch_bounds__(RB) void beamspace_capon_kernel(const BeamspaceLaun
           S[b].y = fmaf(xv.x, tw.y, fmaf(xv.y, tw.x, S[b].y));
         }
       }
-      for (int k = 0; k < K; ++k) {
-        if (k > 0) {
-          const float2 xin = x[k + L - 1], xout = x[k - 1];
-          const float2 dlt = make_float2(xin.x - xout.x, xin.y - xout.y);
+    }
+    auto accumulate = [&]() {
+      float2 v[NB];
 #pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            const float2 u = make_float2(S[b].x + dlt.x, S[b].y + dlt.y);
-            const float2 om = a.omega[b];
-            S[b] = make_float2(fmaf(u.x, om.x, -u.y * om.y), fmaf(u.x, om.y, u.y * om.x));
-          }
-        }
-        float2 v[NB];
+      for (int b = 0; b < NB; ++b) {
+        v[b] = make_float2(S[b].x * rsL, S[b].y * rsL);
+        sm[b].x += v[b].x;
+        sm[b].y += v[b].y;
+      }
+      int e = 0;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          v[b] = make_float2(S[b].x * rsL, S[b].y * rsL);
-          sm[b].x += v[b].x;
-          sm[b].y += v[b].y;
-        }
-        int e = 0;
+      for (int b1 = 0; b1 < NB; ++b1) {
 #pragma unroll
-        for (int b1 = 0; b1 < NB; ++b1) {
-#pragma unroll
-          for (int b2 = b1; b2 < NB; ++b2, ++e) {   // G[b1][b2] += v[b1] conj(v[b2])
-            G[e].x = fmaf(v[b1].x, v[b2].x, fmaf(v[b1].y, v[b2].y, G[e].x));
-            G[e].y = fmaf(v[b1].y, v[b2].x, fmaf(-v[b1].x, v[b2].y, G[e].y));
-          }
+        for (int b2 = b1; b2 < NB; ++b2, ++e) {   // G[b1][b2] += v[b1] conj(v[b2])
+          G[e].x = fmaf(v[b1].x, v[b2].x, fmaf(v[b1].y, v[b2].y, G[e].x));
+          G[e].y = fmaf(v[b1].y, v[b2].x, fmaf(-v[b1].x, v[b2].y, G[e].y));
         }
       }
+    };
+    accumulate();
+    // S_k = (S_{k-1} + x[k+L-1] - x[k-1]) omega, k = 1 .. K-1: windows of the outgoing
+    // (tail) and incoming (head) elements
+    for (int k0 = 1; k0 < K; k0 += BS_W) {
+      bs_stage(tail, xw, nrows, a.Mstride, M, k0 - 1, lane);
+      bs_stage(head, xw, nrows, a.Mstride, M, k0 + L - 1, lane);
+      for (int i = 0; i < BS_W && k0 + i < K; ++i) {
+        const float2 xin = mine_h[i], xout = mine_t[i];
+        const float2 dlt = make_float2(xin.x - xout.x, xin.y - xout.y);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const float2 u = make_float2(S[b].x + dlt.x, S[b].y + dlt.y);
+          const float2 om = a.omega[b];
+          S[b] = make_float2(fmaf(u.x, om.x, -u.y * om.y), fmaf(u.x, om.y, u.y * om.x));
+        }
+        accumulate();
+      }
     }
-#pragma unroll
-    for (int e = 0; e < NT; ++e) gram[r][e] = G[e];
-    const float invK = 1.f / (float)K;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) smean[r][b] = make_float2(sm[b].x * invK, sm[b].y * invK);
   }
+  __syncthreads();   // the windows are dead; the same LDS holds the Grams
+#pragma unroll
+  for (int e = 0; e < NT; ++e) gram[r][e] = G[e];
+  const float invK = 1.f / (float)K;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) smean[r][b] = make_float2(sm[b].x * invK, sm[b].y * invK);
   __syncthreads();
 
   // ---------------- phase 2: one pixel per thread ----------------
@@ -216,7 +255,7 @@ __global__ __launch_bounds__(RB) void beamspace_capon_kernel(const BeamspaceLaun
 
 template <int NB>
 static hipError_t launch_bs(const BeamspaceLaunch& a, hipStream_t s) {
-  constexpr int RB = NB <= 5 ? 256 : 128;
+  constexpr int RB = 128;
   const int TP = RB - 2 * a.Y;
   if (TP < 1) return hipErrorInvalidValue;
   const int Nzp = a.Nz - 2 * a.Y;

@@ -3,7 +3,7 @@
 
 namespace cosk {
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, bool BWD>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s);
 
 template <int G, int CPL>
@@ -43,10 +43,21 @@ hipError_t launch_capon_bwd(const CaponLaunch& a, int Lmax, int Mmax, hipStream_
     return hipErrorInvalidValue;
   }
   const bool w5 = 2 * a.Y + 1 <= 5;
-  if (Lmax <= 16)
-    return w5 ? launch_one<float, 16, 1, 5, false, false, true>(a, Mmax, s) : launch_one<float, 16, 1, 0, false, false, true>(a, Mmax, s);
-  if (Lmax <= 32)
-    return w5 ? launch_one<float, 16, 2, 5, false, false, true>(a, Mmax, s) : launch_one<float, 16, 2, 0, false, false, true>(a, Mmax, s);
+  if (Mmax > 128) {
+    if (why) *why = "the adjoint supports M <= 128";
+    return hipErrorInvalidValue;
+  }
+  // per-lane k/m slots sized for M <= 64 or M <= 128
+  if (Lmax <= 16) {
+    if (Mmax <= 64)
+      return w5 ? launch_one<float, 16, 1, 5, false, false, 64>(a, Mmax, s) : launch_one<float, 16, 1, 0, false, false, 64>(a, Mmax, s);
+    return w5 ? launch_one<float, 16, 1, 5, false, false, 128>(a, Mmax, s) : launch_one<float, 16, 1, 0, false, false, 128>(a, Mmax, s);
+  }
+  if (Lmax <= 32) {
+    if (Mmax <= 64)
+      return w5 ? launch_one<float, 16, 2, 5, false, false, 64>(a, Mmax, s) : launch_one<float, 16, 2, 0, false, false, 64>(a, Mmax, s);
+    return w5 ? launch_one<float, 16, 2, 5, false, false, 128>(a, Mmax, s) : launch_one<float, 16, 2, 0, false, false, 128>(a, Mmax, s);
+  }
   if (why) *why = "the adjoint supports L <= 32";
   return hipErrorInvalidValue;
 }

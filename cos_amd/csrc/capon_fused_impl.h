@@ -44,8 +44,8 @@ __device__ __forceinline__ cplx<T> ldx(const float2* p) {
   return {(T)v.x, (T)v.y};
 }
 
-// Largest K = M - L + 1 the adjoint epilogue handles (its per-lane k and m slots).
-constexpr int BWD_KMAX = 128;
+// BWD (template argument): 0 for the forward kernel; for the adjoint, the largest M (and
+// so K = M - L + 1) its epilogue handles with per-lane k and m slots (64 or 128).
 
 // Rows of each Cholesky step whose multiplier U[k][i] is broadcast from registers
 // rather than read back from LDS.
@@ -89,7 +89,7 @@ template <typename T, int G, int CPL, int TM> struct OccHint {
                                               : (G == 16 && CPL == 1 ? 2 : 1);
 };
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool BWD = false>
+template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0>
 __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, const int GREG,
                                             const int gid0, const int npix) {
   using Gp = Grp<G>;
@@ -591,7 +591,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           for (int m = g; m < MP; m += G)
             xr0[t * MP + m] = (m < M) ? srcw[(size_t)t * a.Mstride + m] : make_float2(0.f, 0.f);
         __syncthreads();
-        constexpr int KS = (BWD_KMAX + G - 1) / G;
+        constexpr int KS = (BWD + G - 1) / G;
         for (int t = 0; t < TW; ++t) {
           const float2* xt = xr0 + t * MP;
           cplx<T> bacc[KS];
@@ -659,7 +659,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   }
 }
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, bool BWD = false>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
                           amdgpu_waves_per_eu(BWD ? 1 : OccHint<T, G, CPL, TM>::value)))
 void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
@@ -679,7 +679,7 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
 }
 
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, bool BWD = false>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   constexpr int NG = 64 / G, LMAX = G * CPL;
   const int MP = ((Mmax + LMAX + 4 + 3) / 4) * 4;   // zero padding covers every lag read
@@ -689,9 +689,9 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   const int upk = (LMAX * (LMAX + 1) / 2 + 2 + 2 * LMAX) * (int)(sizeof(cplx<T>) / sizeof(float2)) +
                   LMAX * (int)sizeof(T) / (int)sizeof(float2);
   int GREG = TW * MP > upk ? TW * MP : upk;
-  if (BWD) {   // window rows plus beta[K <= BWD_KMAX] for the adjoint epilogue
-    if (Mmax > BWD_KMAX) return hipErrorInvalidValue;
-    const int need = TW * MP + BWD_KMAX * (int)(sizeof(cplx<T>) / sizeof(float2));
+  if (BWD) {   // window rows plus beta[K <= BWD] for the adjoint epilogue
+    if (Mmax > BWD) return hipErrorInvalidValue;
+    const int need = TW * MP + BWD * (int)(sizeof(cplx<T>) / sizeof(float2));
     GREG = GREG > need ? GREG : need;
   }
   const size_t lds = (size_t)NG * GREG * sizeof(float2);

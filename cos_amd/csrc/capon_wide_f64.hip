@@ -6,7 +6,8 @@
 // lane c owns column c, row k of U is published in LDS and read back as a broadcast.
 //
 // Semantics (as in capon_fused_impl.h; paths relative to the reference root):
-//   covariance S[r][c] = sum_t sum_{k<K} x_t[k+r] conj(x_t[k+c]), direct sums
+//   covariance S[r][c] = sum_t sum_{k<K} x_t[k+r] conj(x_t[k+c]) in FP64: the first row
+//              by direct sums, the rest by the displacement recursion along diagonals
 //              (framework/beamformer/capon/RealTimeCapon/buildR_kernel.cu:45-161),
 //              loading d*tr(S)/L (tr = sum of real parts)
 //   solve      Cholesky U^H U = S_L, y = S_L^-1 a with a = 1
@@ -65,19 +66,34 @@ __global__ __launch_bounds__(64) void capon_wide_f64_kernel(const CaponLaunch a,
     }
     __syncthreads();
 
-    // covariance, one packed entry per lane per pass
-    const int NE = L * (L + 1) / 2;
-    for (int e = lane; e < NE; e += 64) {
-      const int c = packed_col(e), r = e - c * (c + 1) / 2;
+    // covariance: first row by direct sums, then the displacement recursion
+    // S[r][c] = S[r-1][c-1] + sum_t (x_t[K-1+r] conj(x_t[K-1+c]) - x_t[r-1] conj(x_t[c-1])),
+    // lane c owning column c (rows r <= c); exact in FP64 to rounding
+    const int c = lane;
+    const int ucol = c * (c + 1) / 2;
+    if (c < L) {
       cplx<double> acc = {0.0, 0.0};
       for (int t = 0; t < TW; ++t) {
         const float2* xr = xs + t * MP;
         for (int k = 0; k < K; ++k) {
-          const float2 u = xr[k + r], v = xr[k + c];
+          const float2 u = xr[k], v = xr[k + c];
           add_mul_conj(acc, cplx<double>{u.x, u.y}, cplx<double>{v.x, v.y});
         }
       }
-      A[e] = acc;
+      A[ucol] = acc;
+    }
+    for (int r = 1; r < L; ++r) {
+      __syncthreads();   // row r-1 is complete
+      if (c >= r && c < L) {
+        cplx<double> acc = A[(r - 1) + (c - 1) * c / 2];
+        for (int t = 0; t < TW; ++t) {
+          const float2* xr = xs + t * MP;
+          const float2 u1 = xr[K - 1 + r], a1 = xr[K - 1 + c], u0 = xr[r - 1], b0 = xr[c - 1];
+          add_mul_conj(acc, cplx<double>{u1.x, u1.y}, cplx<double>{a1.x, a1.y});
+          sub_mul_conj(acc, cplx<double>{u0.x, u0.y}, cplx<double>{b0.x, b0.y});
+        }
+        A[r + ucol] = acc;
+      }
     }
     __syncthreads();
 
@@ -89,19 +105,19 @@ __global__ __launch_bounds__(64) void capon_wide_f64_kernel(const CaponLaunch a,
     // optional materialised R in the reference layout: mem[r*L+c] = conj(S_L[r][c])
     if (a.R && valid) {
       float2* Rp = a.R + (size_t)p * a.Rstride;
+      const int NE = L * (L + 1) / 2;
       for (int e = lane; e < NE; e += 64) {
-        const int c = packed_col(e), r = e - c * (c + 1) / 2;
+        const int cc = packed_col(e), rr = e - cc * (cc + 1) / 2;
         cplx<double> v = A[e];
-        if (r == c) v.x += delta;
-        Rp[r * L + c] = make_float2((float)v.x, (float)-v.y);
-        if (r < c) Rp[c * L + r] = make_float2((float)v.x, (float)v.y);
+        if (rr == cc) v.x += delta;
+        Rp[rr * L + cc] = make_float2((float)v.x, (float)-v.y);
+        if (rr < cc) Rp[cc * L + rr] = make_float2((float)v.x, (float)v.y);
       }
     }
     if (a.cov_only) continue;
 
     // Cholesky U^H U = S + delta I, right-looking; lane c owns column c
     bool notpd = false;
-    const int ucol = lane * (lane + 1) / 2;
     for (int k = 0; k < L; ++k) {
       double piv = A[k + k * (k + 1) / 2].x + delta;
       if (!(piv > 1.0e-300)) {

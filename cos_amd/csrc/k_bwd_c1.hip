@@ -2,6 +2,8 @@
 #include "capon_fused_impl.h"
 
 namespace cosk {
-template hipError_t launch_one<float, 16, 1, 5, false, false, true>(const CaponLaunch&, int, hipStream_t);
-template hipError_t launch_one<float, 16, 1, 0, false, false, true>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<float, 16, 1, 5, false, false, 64>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<float, 16, 1, 5, false, false, 128>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<float, 16, 1, 0, false, false, 64>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<float, 16, 1, 0, false, false, 128>(const CaponLaunch&, int, hipStream_t);
 }  // namespace cosk
