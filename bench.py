@@ -164,7 +164,7 @@ def main():
                    "fallback_pixels": nfb},
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-                     "traffic": measured_traffic(), "kernel": "capon_kernel<float,16,2,5>",
+                     "traffic": measured_traffic(), "kernel": "capon_kernel<float,16,2,5,false,false,0>",
                      "kernel_ms": round(kern_ms, 4), "flop_per_pixel": FLOP_PER_PX},
         "accuracy": {"P_rel_err_max": float(err.max()), "P_rel_err_p999": float(np.quantile(err, 0.999)),
                      "sample": "256 beams x rows 900..915 vs FP64 oracle"},
