@@ -141,7 +141,8 @@ class CaponPlan:
             self._h = None
 
     def run(self, x, d, L, Yavg, *, M=None, off=0, precision="fp32", want=("P", "z", "w"),
-            rows=None, steer_sin=None, cond_thresh=1e6, out=None, stream=None, graph=False):
+            rows=None, steer_sin=None, cond_thresh=1e6, out=None, stream=None, graph=False,
+            fb_avg=False, z_mode="amplitude", p_mode="loaded"):
         """Run one frame.
 
         x: complex64 CUDA tensor [Nx][Nz][Mstride].  rows: optional dict of CUDA tensors
@@ -149,7 +150,10 @@ class CaponPlan:
         precision: "fp32", "fp64" or "fp32+fallback".  steer_sin: optional float32 CUDA
         tensor of sin(theta) values.  graph: capture the frame's launches in a hipGraph
         on the first call and replay it while the descriptor and all tensors are the same
-        (pass the same ``out`` dict to keep the pointers).  Returns a dict of output tensors.
+        (pass the same ``out`` dict to keep the pointers).  fb_avg: forward-backward
+        averaged covariance; z_mode: "amplitude" (w^H g) or "power" (power Capon);
+        p_mode: "loaded" (1/(a^H R_L^-1 a)) or "unloaded" (|w^H R_hat w|).  Returns a dict
+        of output tensors.
         """
         import torch
 
@@ -164,6 +168,9 @@ class CaponPlan:
         ds.L, ds.off, ds.d = int(L), int(off), float(d)
         ds.precision, ds.cond_thresh = prec, float(cond_thresh)
         ds.graph = int(bool(graph))
+        ds.fb_avg = int(bool(fb_avg))
+        ds.z_mode = {"amplitude": 0, "power": 1}[z_mode]
+        ds.p_mode = {"loaded": 0, "unloaded": 1}[p_mode]
         Lmax = int(L)
         keep = []
         if rows is not None:

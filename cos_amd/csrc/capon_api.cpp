@@ -146,6 +146,9 @@ cosk::CaponLaunch base_launch(const cos_capon_desc& ds, const void* x) {
   a.cond_thresh = ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6f;
   a.steer_sin = ds.steer_sin;
   a.nsteer = ds.nsteer;
+  a.fb = ds.fb_avg ? 1 : 0;
+  a.zmode = ds.z_mode;
+  a.pmode = ds.p_mode;
   return a;
 }
 
@@ -208,6 +211,10 @@ int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
   if (ds.nsteer < 0 || (ds.nsteer > 0 && !ds.steer_sin)) return fail(hipErrorInvalidValue, "bad steering");
   if (ds.precision == COS_PREC_FP32_FALLBACK && ds.nsteer > 0)
     return fail(hipErrorInvalidValue, "FP64 fallback does not support steering");
+  if (ds.z_mode != COS_Z_AMPLITUDE && ds.z_mode != COS_Z_POWER) return fail(hipErrorInvalidValue, "bad z_mode");
+  if (ds.p_mode != COS_P_LOADED && ds.p_mode != COS_P_UNLOADED) return fail(hipErrorInvalidValue, "bad p_mode");
+  if (ds.nsteer > 0 && (ds.z_mode != COS_Z_AMPLITUDE || ds.p_mode != COS_P_LOADED))
+    return fail(hipErrorInvalidValue, "steering takes the default z and P definitions");
   return 0;
 }
 
@@ -369,6 +376,8 @@ int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, in
   const cos_capon_desc& ds = *desc;
   if (ds.row_L || ds.row_M || ds.row_off || ds.row_d || ds.nsteer)
     return fail(hipErrorInvalidValue, "beamspace path takes uniform L, M, d and no steering");
+  if (ds.fb_avg || ds.z_mode || ds.p_mode)
+    return fail(hipErrorInvalidValue, "beamspace path takes the default options");
   if (int e = check_args(ds.d, ds.L, ds.Yavg, ds.M, ds.Nx, ds.Nz, Nb)) return e;
   if (Nb < 1 || Nb > 8) return fail(hipErrorInvalidValue, "beamspace path supports 1 <= Nb <= 8");
   if (ds.off < 0 || ds.off + ds.M > ds.Mstride) return fail(hipErrorInvalidValue, "off + M > Mstride");
@@ -417,6 +426,8 @@ int cos_capon_backward(cos_capon_plan* plan, const cos_capon_desc* desc, const c
   cos_capon_desc ds = *desc;
   ds.graph = 0;
   if (ds.nsteer) return fail(hipErrorInvalidValue, "the adjoint does not take steering vectors");
+  if (ds.fb_avg || ds.z_mode || ds.p_mode)
+    return fail(hipErrorInvalidValue, "the adjoint is defined for the default P (no averaging options)");
   if (int e = validate(ds, nullptr, nullptr, 0)) return e;
   const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
   const int Lmax = tables ? ds.Lmax : ds.L;

@@ -9,6 +9,8 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s);
 template <int G, int CPL>
 static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
   const int TW = 2 * a.Y + 1;
+  // the optional variants live in the runtime-window kernels
+  if (a.fb || a.zmode || a.pmode) return launch_one<float, G, CPL, 0, false, false, 0>(a, Mmax, s);
   if (TW <= 5) return launch_one<float, G, CPL, 5, false, false, false>(a, Mmax, s);
   if (TW <= 9) return launch_one<float, G, CPL, 9, false, false, false>(a, Mmax, s);
   return launch_one<float, G, CPL, 0, false, false, false>(a, Mmax, s);

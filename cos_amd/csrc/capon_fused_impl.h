@@ -94,6 +94,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
                                             const int gid0, const int npix) {
   using Gp = Grp<G>;
   constexpr int NG = Gp::NG, LMAX = G * CPL, NS = G * CPL * (CPL + 1) / 2;
+  // the reference's optional variants (forward-backward averaging, power-Capon amplitude,
+  // unloaded power) are compiled into the runtime-window forward kernels only, so the
+  // fixed-window kernels keep their registers
+  constexpr bool OPTS = TM == 0 && !BWD;
   constexpr int TMR = TM > 0 ? TM : 1;
   extern __shared__ float2 xs_all[];
 
@@ -317,6 +321,58 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   }
   const T trS = Gp::sum(trp);
   const T delta = (T)dload * trS / (T)L;
+  const T invK = T(1) / (T)K;
+  const bool steered = a.nsteer > 0;
+  // subarray mean of the centre snapshot, g_c = (1/K) sum_k x_c[k+c], for this lane's
+  // columns (computed before the rows are overwritten: early when the averaging option
+  // reuses the LDS region, otherwise just before the factorisation)
+  cplx<T> gS[CPL];
+  auto subarray_means = [&]() {
+    static_for<0, CPL>([&](auto S_) {
+      constexpr int S = decltype(S_)::value;
+      cplx<T> acc = {T(0), T(0)};
+      if (!steered && cs[S] < L) {
+        const float2* xc = xs + Y * MP + cs[S];
+        for (int k = 0; k < K; ++k) {
+          const cplx<T> v = ldx<T>(xc + k);
+          acc.x += v.x;
+          acc.y += v.y;
+        }
+      }
+      gS[S] = {acc.x * invK, acc.y * invK};
+    });
+  };
+  if constexpr (OPTS) subarray_means();
+  // The snapshot rows are not read past this point.
+  constexpr int NU = LMAX * (LMAX + 1) / 2;
+
+  // Forward-backward averaging S <- (S + J S^T J)/2 (CAP/getCaponAllC.pyx:357-358):
+  // entry (r, c) averages with (L-1-c, L-1-r).  The upper triangle goes through the
+  // group's LDS region; the trace is unchanged.
+  if (OPTS && a.fb) {
+    cplx<T>* const Sp = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
+    static_for<0, CPL>([&](auto S_) {
+      constexpr int S = decltype(S_)::value;
+      const int uc = cs[S] * (cs[S] + 1) / 2;
+      static_for<0, G * (S + 1)>([&](auto R_) {
+        constexpr int R = decltype(R_)::value;
+        Sp[R <= cs[S] ? R + uc : NU] = A[slot<G, S, R>()];
+      });
+    });
+    static_for<0, CPL>([&](auto S_) {
+      constexpr int S = decltype(S_)::value;
+      const int rp = L - 1 - cs[S];
+      static_for<0, G * (S + 1)>([&](auto R_) {
+        constexpr int R = decltype(R_)::value;
+        const int cp = L - 1 - R;
+        const bool in = R <= cs[S] && cs[S] < L;
+        const cplx<T> pv = Sp[in ? rp + cp * (cp + 1) / 2 : NU];
+        cplx<T>& e = A[slot<G, S, R>()];
+        if (in) e = {T(0.5) * (e.x + pv.x), T(0.5) * (e.y + pv.y)};
+      });
+    });
+  }
+
   if (L < LMAX) {
     static_for<0, CPL>([&](auto S_) {
       constexpr int S = decltype(S_)::value;
@@ -347,23 +403,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 
   if (a.cov_only) return;
 
-  const T invK = T(1) / (T)K;
-  const bool steered = a.nsteer > 0;
-  // subarray mean of the centre snapshot, g_c = (1/K) sum_k x_c[k+c], for this lane's columns
-  cplx<T> gS[CPL];
-  static_for<0, CPL>([&](auto S_) {
-    constexpr int S = decltype(S_)::value;
-    cplx<T> acc = {T(0), T(0)};
-    if (!steered && cs[S] < L) {
-      const float2* xc = xs + Y * MP + cs[S];
-      for (int k = 0; k < K; ++k) {
-        const cplx<T> v = ldx<T>(xc + k);
-        acc.x += v.x;
-        acc.y += v.y;
-      }
-    }
-    gS[S] = {acc.x * invK, acc.y * invK};
-  });
+  if constexpr (!OPTS) subarray_means();
 
   // ---------------- Cholesky U^H U = S_L, fused forward solve U^H v = a ----------------
   // LDS region of the group (free now: every read of the snapshots is complete), in
@@ -376,7 +416,6 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   //                         while row k's updates still read the other one
   //   invd (T) at the end   1/U_kk
   // Row k is written to Upk as it is finished, so the registers of A die row by row.
-  constexpr int NU = LMAX * (LMAX + 1) / 2;
   // FP32 16-lane groups broadcast U[k][i] inside the FMAs (v_fmac_f32 with a DPP source);
   // other shapes publish row k in LDS
   constexpr bool DPP_FMA = std::is_same<T, float>::value && G == 16;
@@ -460,7 +499,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     notpd = !(maxinv < T(0.5) * rsqrt_t<T>(tiny_value<T>()));
   }
 
-  const bool want_y = a.w || a.z || BWD;
+  const bool want_y = a.w || a.z || BWD || (OPTS && a.pmode == 1 && a.P);
   const T norm = (T)K * (T)TW;
   const int nrhs = steered ? a.nsteer : 1;
   const float2* xcg = a.x + ((size_t)ix * a.Nz + (j + Y)) * a.Mstride + off;   // centre row (steering)
@@ -514,7 +553,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 
     const size_t po = (size_t)p * nrhs + ia;
     const float nanv = __builtin_nanf("");
-    if (active && g == 0 && a.P) a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
+    if (active && g == 0 && a.P && (!OPTS || a.pmode == 0)) a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
 
     if (want_y) {
       // back solve U y = v, column-oriented: y_j = c_j / U_jj, then c_i -= U[i][j] y_j
@@ -563,6 +602,35 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       });
       zp.x = Gp::sum(zp.x);
       zp.y = Gp::sum(zp.y);
+      if (OPTS && (a.zmode == 1 || a.pmode == 1)) {
+        // w of every column, for the power-Capon amplitude and |w|^2
+        cplx<T>* const wb = Upk + NU + 2;
+        T wn = T(0);
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          const cplx<T> wv = cs[S] < L ? cplx<T>{y[S].x * sc, y[S].y * sc} : cplx<T>{T(0), T(0)};
+          wb[cs[S]] = wv;
+          wn = fma(wv.x, wv.x, fma(wv.y, wv.y, wn));
+        });
+        wn = Gp::sum(wn);
+        if (a.pmode == 1 && active && g == 0 && a.P) {
+          // |w^H R_hat w| with R_hat = S / norm unloaded: (w^H S_L w - delta |w|^2) / norm,
+          // w^H S_L w = 1 / (a^H S_L^-1 a)   (CAP/getCaponMatlab_ref.m:109)
+          a.P[po] = valid ? (float)(fabs(T(1) / accP - delta * wn) / norm) : nanv;
+        }
+        if (a.zmode == 1) {
+          // power Capon: sqrt(sum_k |w^H u_{c,k}|^2) / K over the subarrays of the centre
+          // snapshot (RTC/capon_kernel.cu:235-335, the compiled-out power_capon_kernel)
+          T pk = T(0);
+          for (int k = g; k < K; k += G) {
+            cplx<T> sk = {T(0), T(0)};
+            for (int l = 0; l < L; ++l) add_conj_mul(sk, wb[l], ldx<T>(xcg + k + l));
+            pk = fma(sk.x, sk.x, fma(sk.y, sk.y, pk));
+          }
+          pk = Gp::sum(pk);
+          zp = {sqrt(pk) / (T)K, T(0)};
+        }
+      }
       if (active && g == 0 && a.z)
         a.z[po] = valid ? make_float2((float)zp.x, (float)zp.y) : make_float2(nanv, nanv);
 

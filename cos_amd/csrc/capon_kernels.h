@@ -40,6 +40,8 @@ struct CaponLaunch {
   // steering (optional): nsteer sines; outputs P/z are [p][nsteer], w is [p][nsteer][wstride]
   const float* steer_sin;
   int nsteer;
+  // options: forward-backward averaging, power-Capon amplitude, unloaded power
+  int fb, zmode, pmode;
 };
 
 // Returns hipSuccess or an error; `why` receives a static message on argument errors.

@@ -97,6 +97,25 @@ __global__ __launch_bounds__(64) void capon_wide_f64_kernel(const CaponLaunch a,
     }
     __syncthreads();
 
+    // forward-backward averaging (CAP/getCaponAllC.pyx:357-358): entry (r, c) and its
+    // partner (L-1-c, L-1-r) both become their mean; the pair is handled by the lane of
+    // the smaller column (ties: smaller row), so no entry is written twice
+    if (a.fb) {
+      if (c < L) {
+        for (int r = 0; r <= c; ++r) {
+          const int rp = L - 1 - c, cp = L - 1 - r;
+          if (c < cp || (c == cp && r < rp)) {
+            const int e1 = r + ucol, e2 = rp + cp * (cp + 1) / 2;
+            const cplx<double> u = A[e1], v = A[e2];
+            const cplx<double> m = {0.5 * (u.x + v.x), 0.5 * (u.y + v.y)};
+            A[e1] = m;
+            A[e2] = m;
+          }
+        }
+      }
+      __syncthreads();
+    }
+
     double tp = 0.0;
     if (lane < L) tp = A[lane + lane * (lane + 1) / 2].x;
     const double trS = Grp<64>::sum(tp);
@@ -157,9 +176,9 @@ __global__ __launch_bounds__(64) void capon_wide_f64_kernel(const CaponLaunch a,
     }
     const double norm = (double)K * (double)TW;
     const float nanv = __builtin_nanf("");
-    if (lane == 0 && a.P) a.P[p] = valid ? (float)(1.0 / (norm * accP)) : nanv;
+    if (lane == 0 && a.P && a.pmode == 0) a.P[p] = valid ? (float)(1.0 / (norm * accP)) : nanv;
 
-    if (a.w || a.z) {
+    if (a.w || a.z || (a.pmode == 1 && a.P)) {
       // back solve U y = v, column-oriented
       cplx<double> cr = vv, y = {0.0, 0.0};
       for (int jj = L - 1; jj >= 0; --jj) {
@@ -190,6 +209,29 @@ __global__ __launch_bounds__(64) void capon_wide_f64_kernel(const CaponLaunch a,
       }
       zp.x = Grp<64>::sum(zp.x);
       zp.y = Grp<64>::sum(zp.y);
+      if (a.pmode == 1 || a.zmode == 1) {
+        const cplx<double> wl = lane < L ? wv : cplx<double>{0.0, 0.0};
+        const double wn = Grp<64>::sum(fma(wl.x, wl.x, wl.y * wl.y));
+        // |w^H R_hat w| = |w^H S_L w - delta |w|^2| / norm, w^H S_L w = 1/(a^H S_L^-1 a)
+        if (a.pmode == 1 && lane == 0 && a.P)
+          a.P[p] = valid ? (float)(fabs(1.0 / accP - delta * wn) / norm) : nanv;
+        if (a.zmode == 1) {
+          // power Capon: sqrt(sum_k |w^H u_{c,k}|^2) / K (RTC/capon_kernel.cu:235-335)
+          __syncthreads();
+          cplx<double>* const wb = A;   // the matrix is no longer needed
+          wb[lane] = wl;
+          __syncthreads();
+          double pk = 0.0;
+          const float2* xc = xs + Y * MP;
+          for (int k = lane; k < K; k += 64) {
+            cplx<double> sk = {0.0, 0.0};
+            for (int l = 0; l < L; ++l) add_conj_mul(sk, wb[l], cplx<double>{xc[k + l].x, xc[k + l].y});
+            pk = fma(sk.x, sk.x, fma(sk.y, sk.y, pk));
+          }
+          pk = Grp<64>::sum(pk);
+          zp = {sqrt(pk) / (double)K, 0.0};
+        }
+      }
       if (lane == 0 && a.z) a.z[p] = valid ? make_float2((float)zp.x, (float)zp.y) : make_float2(nanv, nanv);
     }
     if (a.flags && lane == 0) {

@@ -94,7 +94,24 @@ typedef struct {
     int nsteer;
     int graph;           /* nonzero: capture the frame's launches in a hipGraph on the first
                             call and replay it while descriptor and pointers are unchanged */
+    /* Options of the reference's CPU path (framework/beamformer/capon/getCaponAllC.pyx)
+     * and compiled-out GPU variants; 0 selects the reference GPU default. */
+    int fb_avg;          /* forward-backward averaging R <- (R + J R^T J)/2 before loading
+                            (getCaponAllC.pyx:357-358) */
+    int z_mode;          /* COS_Z_*: amplitude output */
+    int p_mode;          /* COS_P_*: power output */
 } cos_capon_desc;
+
+enum {
+    COS_Z_AMPLITUDE = 0, /* z = w^H g, coherent sum over subarrays (RealTimeCapon/capon_kernel.cu:338-427) */
+    COS_Z_POWER = 1      /* z = sqrt(sum_k |w^H u_{c,k}|^2) / K, the power-Capon amplitude of the
+                            compiled-out power_capon_kernel (RealTimeCapon/capon_kernel.cu:235-335) */
+};
+enum {
+    COS_P_LOADED = 0,    /* P = 1/(a^H R_L^-1 a) = w^H R_L w */
+    COS_P_UNLOADED = 1   /* P = |w^H R_hat w| with the unloaded R_hat, the definition of
+                            framework/beamformer/capon/getCaponMatlab_ref.m:109 */
+};
 
 int cos_capon_plan_create(cos_capon_plan** plan);
 int cos_capon_plan_destroy(cos_capon_plan* plan);

@@ -178,6 +178,49 @@ def capon(x: np.ndarray, d: float, L: int, Yavg: int, pixels=None, ref_fp32_cons
     return out
 
 
+def fb_average(R: np.ndarray) -> np.ndarray:
+    """Forward-backward averaging (R + J R^T J)/2, J the exchange matrix
+    (CAP/getCaponAllC.pyx:357-358, applied to the normalised covariance before loading)."""
+    return 0.5 * (R + np.flip(np.swapaxes(R, -1, -2), axis=(-2, -1)))
+
+
+def capon_options(x: np.ndarray, d: float, L: int, Yavg: int, pixels=None, fb: bool = False,
+                  z_mode: int = 0, p_mode: int = 0):
+    """Capon with the reference's optional variants (FP64, a = 1):
+      fb      forward-backward averaged covariance (fb_average) before the loading;
+      z_mode  0: z = w^H g (RTC/capon_kernel.cu:338-427); 1: power Capon
+              z = sqrt(sum_k |w^H u_{c,k}|^2) / K over the centre snapshot's subarrays
+              (RTC/capon_kernel.cu:235-335, power_capon_kernel);
+      p_mode  0: P = 1/(a^H R_L^-1 a); 1: P = |w^H R_hat w| with R_hat unloaded
+              (CAP/getCaponMatlab_ref.m:109).
+    Returns dict P, z, w."""
+    Nx, Nz, M = x.shape
+    K = M - L + 1
+    if pixels is None:
+        pixels = all_pixels(Nx, Nz, Yavg)
+    ix, j = pixels
+    rows = window_rows(x, ix, j, Yavg)
+    Rh = covariance_unnormalised(rows, L) / (K * (2 * Yavg + 1))
+    if fb:
+        Rh = fb_average(Rh)
+    RL = diagonal_load(Rh, float(d), L)
+    y = solve_cholesky(RL, np.ones(RL.shape[:-1], dtype=np.complex128))
+    q = np.real(y.sum(axis=-1))
+    w = y / q[:, None]
+    centre = rows[:, Yavg, :]
+    if z_mode == 0:
+        z = np.sum(np.conj(w) * subarray_mean(centre, L), axis=-1)
+    else:
+        u = subarrays(centre[:, None, :], L)[:, 0]               # [B][K][L]
+        s = np.einsum("bl,bkl->bk", np.conj(w), u)
+        z = np.sqrt(np.sum(np.abs(s) ** 2, axis=-1)) / K + 0j
+    if p_mode == 0:
+        P = 1.0 / q
+    else:
+        P = np.abs(np.einsum("bi,bij,bj->b", np.conj(w), Rh, w))
+    return dict(P=P, z=z, w=w)
+
+
 def reference_gpu_outputs(x: np.ndarray, d: float, L: int, Yavg: int, Nb: int = 0):
     """What Capon::getCapon returns for the whole cube (RTC/Capon.cpp:73-463 with the
     default build: COPY_R_DEVICE_HOST on, SLIDING_BEAMSPACE off): z [Nx][Nz-2Y],

@@ -389,3 +389,32 @@ def test_beamspace_sonar_slice():
     e = rel(P[sel], ref["P"])
     print(f"sonar P rel err max {e.max():.3e}", flush=True)
     assert e.max() <= 1e-4
+
+
+# ------------------------------------------------------------------ reference options
+
+OPTION_CASES = [
+    # (M, L, Y, precision): 16-lane one column, two columns, 64-lane, FP64 direct, FP64 wide
+    (24, 12, 1, "fp32"), (48, 24, 2, "fp32"), (80, 40, 1, "fp32"), (48, 24, 2, "fp64"), (72, 40, 1, "fp64"),
+]
+
+
+@pytest.mark.parametrize("M,L,Y,prec", OPTION_CASES)
+@pytest.mark.parametrize("fb,zm,pm", [(True, "amplitude", "loaded"), (False, "power", "unloaded"),
+                                      (True, "power", "unloaded")])
+def test_reference_options_match_oracle(M, L, Y, prec, fb, zm, pm):
+    """Forward-backward averaging, power-Capon amplitude and the unloaded power against the
+    FP64 oracle: P and z within 1e-4 (FP32) or 2e-6 (FP64 stored as float32), w within 1e-3
+    of its largest entry (FP32)."""
+    say("options", M, L, Y, prec, fb, zm, pm)
+    x = uniform_cube(M + 7 * L, 3, 12, M)
+    o, _ = _plan_run(x, 0.05, L, Y, precision=prec, fb_avg=fb, z_mode=zm, p_mode=pm)
+    ref = O.capon_options(x.astype(np.complex128), 0.05, L, Y, fb=fb,
+                          z_mode=1 if zm == "power" else 0, p_mode=1 if pm == "unloaded" else 0)
+    tol = 1e-4 if prec == "fp32" else 2e-6
+    eP = rel(o["P"], ref["P"])
+    ez = np.abs(o["z"] - ref["z"]) / np.abs(ref["z"]).max()
+    print(f"P err {eP.max():.2e} z err {ez.max():.2e}", flush=True)
+    assert eP.max() <= tol
+    assert ez.max() <= tol
+    np.testing.assert_allclose(o["w"].reshape(-1, L), ref["w"], atol=(1e-3 if prec == "fp32" else 1e-5) * np.abs(ref["w"]).max())

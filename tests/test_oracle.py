@@ -292,3 +292,66 @@ def test_condition_bound_holds():
         rows = O.window_rows(x, np.array([0]), np.array([1]), 1)
         RL = O.diagonal_load(O.covariance_unnormalised(rows, 8), d, 8)[0]
         assert np.linalg.cond(RL) <= O.loaded_condition_bound(d, 8) * (1 + 1e-9)
+
+
+# ------------------------------------------------------------------ reference options
+
+def test_fb_average_equals_conjugate_reversed_data():
+    """(R + J R^T J)/2 is the covariance of the data together with its conjugate-reversed
+    copy (the backward snapshots J conj(u)): an independent path through the covariance."""
+    rng = np.random.default_rng(21)
+    rows = rng.standard_normal((3, 5, 14)) + 1j * rng.standard_normal((3, 5, 14))
+    L = 6
+    Rf = O.covariance_unnormalised(rows, L)
+    Rb = O.covariance_unnormalised(np.conj(rows[..., ::-1]), L)
+    np.testing.assert_allclose(O.fb_average(Rf), 0.5 * (Rf + Rb), rtol=0, atol=1e-12 * np.abs(Rf).max())
+    F = O.fb_average(Rf)
+    J = np.eye(L)[::-1]
+    np.testing.assert_allclose(F, J @ np.swapaxes(F, -1, -2) @ J, atol=1e-12 * np.abs(F).max())   # persymmetric
+    np.testing.assert_allclose(O.fb_average(F), F, atol=1e-12 * np.abs(F).max())                    # idempotent
+    np.testing.assert_allclose(np.trace(F, axis1=-2, axis2=-1), np.trace(Rf, axis1=-2, axis2=-1), rtol=1e-12)
+
+
+def test_capon_options_default_equals_capon():
+    rng = np.random.default_rng(22)
+    x = rng.standard_normal((2, 9, 12)) + 1j * rng.standard_normal((2, 9, 12))
+    o = O.capon(x, 0.05, 5, 1)
+    q = O.capon_options(x, 0.05, 5, 1)
+    np.testing.assert_allclose(q["P"], o["P"], rtol=1e-12)
+    np.testing.assert_allclose(q["z"], o["z"], rtol=1e-10)
+    np.testing.assert_allclose(q["w"], o["w"], atol=1e-12)
+
+
+def test_power_capon_constant_snapshot_closed_form():
+    """A centre snapshot constant over the aperture gives w^H u_k = c (a^H w = 1) for every
+    subarray, so the power-Capon amplitude is |c| / sqrt(K) whatever the weights."""
+    rng = np.random.default_rng(23)
+    M, L, Y = 16, 6, 1
+    K = M - L + 1
+    x = rng.standard_normal((1, 3, M)) + 1j * rng.standard_normal((1, 3, M))
+    c = 0.7 - 1.3j
+    x[0, 1, :] = c
+    q = O.capon_options(x, 0.1, L, Y, z_mode=1, fb=True)
+    np.testing.assert_allclose(q["z"][0].real, abs(c) / np.sqrt(K), rtol=1e-12)
+    assert abs(q["z"][0].imag) == 0.0
+
+
+def test_unloaded_power_closed_form_and_identity():
+    """Snapshots sqrt(3c) e_t (L = M = 3, Yavg = 1) give R_hat = c I, w = 1/3:
+    |w^H R_hat w| = c/3 and the loaded P = c(1+d)/3.  On random data the unloaded power
+    equals P - (d tr(R_hat)/L) |w|^2."""
+    c, d = 2.5, 0.2
+    x = (np.sqrt(3 * c) * np.eye(3))[None].astype(np.complex128)
+    q0 = O.capon_options(x, d, 3, 1)
+    q1 = O.capon_options(x, d, 3, 1, p_mode=1)
+    np.testing.assert_allclose(q1["P"], [c / 3], rtol=1e-12)
+    np.testing.assert_allclose(q0["P"], [c * (1 + d) / 3], rtol=1e-12)
+    rng = np.random.default_rng(24)
+    x = rng.standard_normal((2, 7, 10)) + 1j * rng.standard_normal((2, 7, 10))
+    L, Y = 4, 2
+    q0 = O.capon_options(x, d, L, Y, fb=True)
+    q1 = O.capon_options(x, d, L, Y, fb=True, p_mode=1)
+    rows = O.window_rows(x, *O.all_pixels(2, 7, Y), Y)
+    Rh = O.fb_average(O.covariance_unnormalised(rows, L) / ((10 - L + 1) * (2 * Y + 1)))
+    tr = np.real(np.trace(Rh, axis1=-2, axis2=-1))
+    np.testing.assert_allclose(q1["P"], q0["P"] - d * tr / L * np.sum(np.abs(q0["w"]) ** 2, axis=-1), rtol=1e-10)
