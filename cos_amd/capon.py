@@ -236,6 +236,37 @@ class CaponPlan:
                                             ptr("w_bs"), ptr("w"), wstride, stream))
         return o
 
+    def run_subspace(self, x, d, L, Yavg, V, *, M=None, off=0, fb_avg=False, want=("P", "z", "w"), out=None,
+                     stream=None):
+        """Capon in the subspace of the rows of V (complex64 CUDA tensor [Nb][L]), loading in the
+        subspace (the reference CPU path's subspace option).  x: complex64 CUDA tensor
+        [Nx][Nz][Mstride].  want: any of P, z, w (w is [npx][Nb]).  Returns a dict of tensors."""
+        import torch
+
+        if not (x.is_cuda and x.dtype == torch.complex64 and x.is_contiguous()):
+            raise ValueError("x must be a contiguous complex64 CUDA tensor")
+        if not (V.is_cuda and V.dtype == torch.complex64 and V.is_contiguous() and V.dim() == 2 and V.shape[1] == L):
+            raise ValueError("V must be a contiguous complex64 CUDA tensor [Nb][L]")
+        Nx, Nz, Mstride = x.shape
+        Nb = V.shape[0]
+        ds = CaponDesc()
+        ds.Nx, ds.Nz, ds.Mstride, ds.Yavg = Nx, Nz, Mstride, int(Yavg)
+        ds.M = int(M if M is not None else Mstride)
+        ds.L, ds.off, ds.d = int(L), int(off), float(d)
+        ds.fb_avg = int(bool(fb_avg))
+        npx = Nx * (Nz - 2 * Yavg)
+        o = dict(out or {})
+        shapes = {"P": ((npx,), torch.float32), "z": ((npx,), torch.complex64), "w": ((npx, Nb), torch.complex64)}
+        for k in want:
+            if k not in o:
+                o[k] = torch.empty(shapes[k][0], dtype=shapes[k][1], device=x.device)
+        ptr = lambda k: o[k].data_ptr() if k in o else None
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device).cuda_stream
+        check(lib().cos_capon_subspace_run(self._h, C.byref(ds), V.data_ptr(), int(Nb), x.data_ptr(), ptr("P"),
+                                           ptr("z"), ptr("w"), stream))
+        return o
+
     def fallback_count(self) -> int:
         c = C.c_int()
         check(lib().cos_capon_last_fallback_count(self._h, C.byref(c)))

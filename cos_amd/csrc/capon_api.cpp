@@ -369,6 +369,39 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   return 0;
 }
 
+int cos_capon_subspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* V, int Nb,
+                           const cos_cf32* x, float* P, cos_cf32* z, cos_cf32* w, void* stream) {
+  g_last_error.clear();
+  if (!plan || !desc || !x || !V) return fail(hipErrorInvalidValue, "plan, desc, V and x must be non-NULL");
+  cos_capon_desc ds = *desc;
+  if (ds.row_L || ds.row_M || ds.row_off || ds.row_d || ds.nsteer)
+    return fail(hipErrorInvalidValue, "subspace path takes uniform L, M, d and no steering");
+  if (ds.z_mode || ds.p_mode) return fail(hipErrorInvalidValue, "subspace path takes the default z and P");
+  if (int e = check_args(ds.d, ds.L, ds.Yavg, ds.M, ds.Nx, ds.Nz, Nb)) return e;
+  if (Nb < 1) return fail(hipErrorInvalidValue, "need 1 <= Nb <= L");
+  if (ds.off < 0 || ds.off + ds.M > ds.Mstride) return fail(hipErrorInvalidValue, "off + M > Mstride");
+  const int L = ds.L;
+  const size_t lds = ((size_t)L * Nb + (size_t)Nb * Nb + 3 * (size_t)Nb + L) * 16 + (size_t)L * L * 8;
+  if (lds > 64 * 1024) return fail(hipErrorInvalidValue, "subspace path: L and Nb too large for one block");
+  const size_t npx = (size_t)ds.Nx * (ds.Nz - 2 * ds.Yavg);
+  hipError_t e;
+  if ((e = plan->R.ensure(npx * L * L * sizeof(cos_cf32))) != hipSuccess) return fail_hip(e, "covariance buffer");
+  hipStream_t s = (hipStream_t)stream;
+  // unloaded, unnormalised covariance of every pixel (FP32), then the subspace solve (FP64)
+  ds.d = 0.f;
+  cosk::CaponLaunch a = base_launch(ds, x);
+  a.R = plan->R.as<float2>();
+  a.Rstride = L * L;
+  a.cov_only = 1;
+  const char* why = nullptr;
+  if ((e = cosk::launch_capon_f32(a, L, ds.M, s, &why)) != hipSuccess)
+    return why ? fail(e, why) : fail_hip(e, "covariance kernel");
+  if ((e = cosk::subspace_capon(plan->R.as<float2>(), (const float2*)V, (const float2*)x, L, Nb, ds.M, ds.Mstride,
+                                ds.off, ds.Yavg, ds.Nx, ds.Nz, desc->d, (float2*)w, (float2*)z, P, s)) != hipSuccess)
+    return fail_hip(e, "subspace kernel");
+  return 0;
+}
+
 int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, int Nb, const cos_cf32* x,
                             float* P, cos_cf32* z, cos_cf32* w_bs, cos_cf32* w, int wstride, void* stream) {
   g_last_error.clear();

@@ -10,6 +10,9 @@ hipError_t compact_flags(const uint8_t* flags, int n, int* list, int* count, hip
 hipError_t solve_lu(const float2* A, const float2* b, float2* x, int N, int batch, hipStream_t s);
 hipError_t buildr_beamspace(const float2* xbs, float2* R, float d, int Nb, int K, int Y, int Nx,
                             int Nz, hipStream_t s);
+// Capon in the subspace of V [Nb][L] from the conj-stored unnormalised S of every pixel.
+hipError_t subspace_capon(const float2* R, const float2* V, const float2* x, int L, int Nb, int M, int Mstride,
+                          int off, int Y, int Nx, int Nz, float d, float2* w, float2* z, float* P, hipStream_t s);
 hipError_t beamspace_matrix(const float2* R, const float2* B, const float2* x, int L, int Nb, int M,
                             int Y, int Nx, int Nz, float2* w, float2* z, float* P, hipStream_t s);
 

@@ -288,6 +288,147 @@ __global__ __launch_bounds__(64) void beamspace_matrix_kernel(const float2* Rg, 
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Capon in the subspace of a user matrix V [Nb][L] (framework/beamformer/capon/
+// getCaponAllC.pyx:230,346-447, loading in the subspace, its default): from the
+// conj-stored unnormalised S of one pixel, R = V (S/norm) V^H, loading d tr(R)/Nb,
+// a = V 1, g = V g_es with g_es[l] = (1/K) sum_k x_c[k+l], y = R_L^-1 a,
+// w = y/(a^H y) (Nb entries), z = w^H g, P = 1/(a^H y).  One 64-thread block per pixel,
+// FP64 accumulation (Nb and L are small).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void subspace_capon_kernel(const float2* Rg, const float2* V,
+                                                            const float2* x, int L, int Nb, int M,
+                                                            int Mstride, int off, int Y, int Nx, int Nz,
+                                                            float d, float2* w, float2* z, float* P) {
+  extern __shared__ double2 smd[];
+  const int Nzp = Nz - 2 * Y, K = M - L + 1;
+  const int p = blockIdx.x;
+  if (p >= Nx * Nzp) return;
+  double2* Tm = smd;              // [L][Nb] (S/norm) V^H
+
+  double2* Rb = Tm + L * Nb;      // [Nb][Nb]
+  double2* ab = Rb + Nb * Nb;     // [Nb] V 1
+  double2* gb = ab + Nb;          // [Nb] V g_es
+  double2* yb = gb + Nb;          // [Nb]
+  double2* ge = yb + Nb;          // [L] g_es
+  float2* S = reinterpret_cast<float2*>(ge + L);   // [L][L] S[r][c] (unnormalised)
+  const double inorm = 1.0 / ((double)K * (2 * Y + 1));
+  const float2* Rp = Rg + (size_t)p * L * L;
+  for (int e = threadIdx.x; e < L * L; e += 64) {
+    const float2 v = Rp[e];   // conj(S[r][c])
+    S[e] = make_float2(v.x, -v.y);
+  }
+  const int ix = p / Nzp, j = p % Nzp;
+  const float2* xc = x + ((size_t)ix * Nz + j + Y) * Mstride + off;
+  for (int l = threadIdx.x; l < L; l += 64) {
+    double sx = 0, sy = 0;
+    for (int k = 0; k < K; ++k) { sx += xc[k + l].x; sy += xc[k + l].y; }
+    ge[l] = make_double2(sx / K, sy / K);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < L * Nb; e += 64) {
+    const int r = e / Nb, bb = e % Nb;
+    double sx = 0, sy = 0;
+    for (int c = 0; c < L; ++c) {   // S[r][c] conj(V[bb][c])
+      const float2 sv = S[r * L + c];
+      const float2 v = V[bb * L + c];
+      sx += (double)sv.x * v.x + (double)sv.y * v.y;
+      sy += (double)sv.y * v.x - (double)sv.x * v.y;
+    }
+    Tm[e] = make_double2(sx * inorm, sy * inorm);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < Nb * Nb; e += 64) {
+    const int b1 = e / Nb, b2 = e % Nb;
+    double sx = 0, sy = 0;
+    for (int r = 0; r < L; ++r) {   // V[b1][r] T[r][b2]
+      const float2 u = V[b1 * L + r];
+      const double2 v = Tm[r * Nb + b2];
+      sx += u.x * v.x - u.y * v.y;
+      sy += u.x * v.y + u.y * v.x;
+    }
+    Rb[e] = make_double2(sx, sy);
+  }
+  for (int bb = threadIdx.x; bb < Nb; bb += 64) {
+    double ax = 0, ay = 0, gx = 0, gy = 0;
+    for (int l = 0; l < L; ++l) {
+      const float2 v = V[bb * L + l];
+      ax += v.x;
+      ay += v.y;
+      gx += v.x * ge[l].x - v.y * ge[l].y;
+      gy += v.x * ge[l].y + v.y * ge[l].x;
+    }
+    ab[bb] = make_double2(ax, ay);
+    gb[bb] = make_double2(gx, gy);
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double tr = 0;
+  for (int i = 0; i < Nb; ++i) tr += Rb[i * Nb + i].x;
+  const double load = (double)d * tr / Nb;
+  // Cholesky U^H U = Rb + load I (upper triangle in place), U^H v = a, U y = v
+  for (int i = 0; i < Nb; ++i) {
+    double dr = Rb[i * Nb + i].x + load;
+    for (int k = 0; k < i; ++k) dr -= Rb[k * Nb + i].x * Rb[k * Nb + i].x + Rb[k * Nb + i].y * Rb[k * Nb + i].y;
+    const double di = sqrt(dr > 1e-300 ? dr : 1e-300);
+    Rb[i * Nb + i] = make_double2(di, 0.0);
+    for (int c = i + 1; c < Nb; ++c) {
+      double sx = Rb[i * Nb + c].x, sy = Rb[i * Nb + c].y;
+      for (int k = 0; k < i; ++k) {   // conj(U[k][i]) U[k][c]
+        const double2 u = Rb[k * Nb + i], v = Rb[k * Nb + c];
+        sx -= u.x * v.x + u.y * v.y;
+        sy -= u.x * v.y - u.y * v.x;
+      }
+      Rb[i * Nb + c] = make_double2(sx / di, sy / di);
+    }
+  }
+  for (int i = 0; i < Nb; ++i) {
+    double sx = ab[i].x, sy = ab[i].y;
+    for (int k = 0; k < i; ++k) {
+      const double2 u = Rb[k * Nb + i], v = yb[k];
+      sx -= u.x * v.x + u.y * v.y;
+      sy -= u.x * v.y - u.y * v.x;
+    }
+    yb[i] = make_double2(sx / Rb[i * Nb + i].x, sy / Rb[i * Nb + i].x);
+  }
+  for (int i = Nb - 1; i >= 0; --i) {
+    double sx = yb[i].x, sy = yb[i].y;
+    for (int k = i + 1; k < Nb; ++k) {
+      const double2 u = Rb[i * Nb + k], v = yb[k];
+      sx -= u.x * v.x - u.y * v.y;
+      sy -= u.x * v.y + u.y * v.x;
+    }
+    yb[i] = make_double2(sx / Rb[i * Nb + i].x, sy / Rb[i * Nb + i].x);
+  }
+  double qx = 0, qy = 0;   // a^H y
+  for (int i = 0; i < Nb; ++i) {
+    qx += ab[i].x * yb[i].x + ab[i].y * yb[i].y;
+    qy += ab[i].x * yb[i].y - ab[i].y * yb[i].x;
+  }
+  const double den = qx * qx + qy * qy;
+  double zx = 0, zy = 0;
+  for (int i = 0; i < Nb; ++i) {
+    const double wx = (yb[i].x * qx + yb[i].y * qy) / den, wy = (yb[i].y * qx - yb[i].x * qy) / den;
+    if (w) w[(size_t)p * Nb + i] = make_float2((float)wx, (float)wy);
+    zx += wx * gb[i].x + wy * gb[i].y;   // conj(w) g
+    zy += wx * gb[i].y - wy * gb[i].x;
+  }
+  if (z) z[p] = make_float2((float)zx, (float)zy);
+  if (P) P[p] = (float)(1.0 / qx);
+}
+
+hipError_t subspace_capon(const float2* R, const float2* V, const float2* x, int L, int Nb, int M, int Mstride,
+                          int off, int Y, int Nx, int Nz, float d, float2* w, float2* z, float* P, hipStream_t s) {
+  const int npx = Nx * (Nz - 2 * Y);
+  if (npx <= 0) return hipSuccess;
+  const size_t lds = ((size_t)L * Nb + (size_t)Nb * Nb + 3 * (size_t)Nb + L) * sizeof(double2) +
+                     (size_t)L * L * sizeof(float2);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(subspace_capon_kernel, dim3(npx), dim3(64), lds, s, R, V, x, L, Nb, M, Mstride, off, Y,
+                     Nx, Nz, d, w, z, P);
+  return hipGetLastError();
+}
+
 hipError_t beamspace_matrix(const float2* R, const float2* B, const float2* x, int L, int Nb, int M,
                             int Y, int Nx, int Nz, float2* w, float2* z, float* P, hipStream_t s) {
   const int npx = Nx * (Nz - 2 * Y);

@@ -136,6 +136,17 @@ int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, in
                             const cos_cf32* x, float* P, cos_cf32* z, cos_cf32* w_bs,
                             cos_cf32* w, int wstride, void* stream);
 
+/* Capon in the subspace spanned by the rows of a user matrix V [Nb][L] (device, row-major):
+ * the reference CPU path's subspace option (framework/beamformer/capon/getCaponAllC.pyx:
+ * 230,346-447, loading d*tr(R)/Nb in the subspace, its default), which its GPU build does
+ * not implement (only Nb is taken from V, framework/beamformer/capon/getCaponCUDA.pyx:33-34).
+ * R = V R_hat V^H with R_hat the (forward-backward averaged when desc.fb_avg) normalised
+ * covariance, a = V 1, g = V g; uses Nx, Nz, Mstride, M, L, off, Yavg, d, fb_avg of desc.
+ * Outputs (device, any may be NULL): P [npx] 1/(a^H R_L^-1 a), z [npx] w^H g,
+ * w [npx][Nb] subspace weights.  Keeps an [npx][L*L] covariance buffer in the plan. */
+int cos_capon_subspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* V, int Nb,
+                           const cos_cf32* x, float* P, cos_cf32* z, cos_cf32* w, void* stream);
+
 /* Adjoint of the power image (no reference counterpart; the differentiable workload).
  * Given dl/dP (grad_P [npx], device), adds dl/dx into grad_x (same layout as x; the caller
  * zeroes it first; accumulated with atomics) and writes the per-pixel dl/dd into grad_d

@@ -221,6 +221,35 @@ def capon_options(x: np.ndarray, d: float, L: int, Yavg: int, pixels=None, fb: b
     return dict(P=P, z=z, w=w)
 
 
+def capon_subspace(x: np.ndarray, d: float, L: int, Yavg: int, V: np.ndarray, pixels=None,
+                   fb: bool = False):
+    """Capon in the subspace spanned by the rows of a user matrix V [Nb][L]
+    (CAP/getCaponAllC.pyx:230,346-447 with DIAGONAL_LOADING_IN_BEAMSPACE, its default):
+    R = V R_hat V^H (R_hat normalised, forward-backward averaged first when fb),
+    loading d tr(R)/Nb, a = V 1, g = V g_es, y = R_L^-1 a, w = y/(a^H y), z = w^H g,
+    P = 1/(a^H R_L^-1 a).  Returns dict P, z, w (Nb-dimensional)."""
+    Nx, Nz, M = x.shape
+    K = M - L + 1
+    if pixels is None:
+        pixels = all_pixels(Nx, Nz, Yavg)
+    ix, j = pixels
+    V = np.asarray(V, dtype=np.complex128)
+    Nb = V.shape[0]
+    rows = window_rows(x, ix, j, Yavg)
+    Rh = covariance_unnormalised(rows, L) / (K * (2 * Yavg + 1))
+    if fb:
+        Rh = fb_average(Rh)
+    Rbs = V @ Rh @ V.conj().T
+    RL = diagonal_load(Rbs, float(d), Nb)
+    a = V @ np.ones(L, dtype=np.complex128)
+    y = solve_cholesky(RL, np.broadcast_to(a, RL.shape[:-1]))
+    q = np.sum(np.conj(a) * y, axis=-1)
+    w = y / q[:, None]
+    g = subarray_mean(rows[:, Yavg, :], L) @ V.T
+    z = np.sum(np.conj(w) * g, axis=-1)
+    return dict(P=1.0 / np.real(q), z=z, w=w)
+
+
 def reference_gpu_outputs(x: np.ndarray, d: float, L: int, Yavg: int, Nb: int = 0):
     """What Capon::getCapon returns for the whole cube (RTC/Capon.cpp:73-463 with the
     default build: COPY_R_DEVICE_HOST on, SLIDING_BEAMSPACE off): z [Nx][Nz-2Y],

@@ -418,3 +418,40 @@ def test_reference_options_match_oracle(M, L, Y, prec, fb, zm, pm):
     assert eP.max() <= tol
     assert ez.max() <= tol
     np.testing.assert_allclose(o["w"].reshape(-1, L), ref["w"], atol=(1e-3 if prec == "fp32" else 1e-5) * np.abs(ref["w"]).max())
+
+
+@pytest.mark.parametrize("M,L,Y,Nb,fb", [(24, 12, 1, 3, False), (48, 24, 2, 6, True), (64, 40, 1, 5, False)])
+def test_subspace_matches_oracle(M, L, Y, Nb, fb):
+    """Capon in the subspace of a random V against the FP64 oracle (FP32 covariance, FP64
+    subspace solve): P within 1e-4, z and w within 1e-4 of their scale."""
+    import torch
+    from cos_amd import CaponPlan
+    say("subspace", M, L, Y, Nb, fb)
+    x = uniform_cube(M + L + Nb, 3, 14, M)
+    rng = np.random.default_rng(Nb)
+    V = ((rng.standard_normal((Nb, L)) + 1j * rng.standard_normal((Nb, L))) / np.sqrt(L)).astype(np.complex64)
+    o = CaponPlan().run_subspace(torch.from_numpy(x).cuda(), 0.05, L, Y, torch.from_numpy(V).cuda(), fb_avg=fb)
+    torch.cuda.synchronize()
+    o = {k: v.cpu().numpy() for k, v in o.items()}
+    ref = O.capon_subspace(x.astype(np.complex128), 0.05, L, Y, V.astype(np.complex128), fb=fb)
+    eP = rel(o["P"], ref["P"])
+    print(f"subspace P err {eP.max():.2e}", flush=True)
+    assert eP.max() <= 1e-4
+    assert np.abs(o["z"] - ref["z"]).max() <= 1e-4 * np.abs(ref["z"]).max()
+    assert np.abs(o["w"] - ref["w"]).max() <= 1e-4 * np.abs(ref["w"]).max()
+
+
+def test_subspace_butler_equals_beamspace():
+    """The Butler rows as V reproduce the sliding-DFT beamspace path (loading in beamspace)."""
+    import torch
+    from cos_amd import CaponPlan
+    say("subspace butler", 32, 16, 1, 5)
+    x = uniform_cube(77, 2, 20, 32)
+    B = O.butler_matrix(5, 16).astype(np.complex64)
+    plan = CaponPlan()
+    xt = torch.from_numpy(x).cuda()
+    s = plan.run_subspace(xt, 0.05, 16, 1, torch.from_numpy(B).cuda())
+    b = plan.run_beamspace(xt, 0.05, 16, 1, 5, want=("P", "z", "w_bs"))
+    torch.cuda.synchronize()
+    assert rel(s["P"].cpu().numpy(), b["P"].cpu().numpy()).max() <= 2e-4
+    np.testing.assert_allclose(s["w"].cpu().numpy(), b["w_bs"].cpu().numpy(), atol=2e-4 * float(b["w_bs"].abs().max()))

@@ -355,3 +355,25 @@ def test_unloaded_power_closed_form_and_identity():
     Rh = O.fb_average(O.covariance_unnormalised(rows, L) / ((10 - L + 1) * (2 * Y + 1)))
     tr = np.real(np.trace(Rh, axis1=-2, axis2=-1))
     np.testing.assert_allclose(q1["P"], q0["P"] - d * tr / L * np.sum(np.abs(q0["w"]) ** 2, axis=-1), rtol=1e-10)
+
+
+def test_subspace_pins():
+    """V = I reduces to element space; a unitary V leaves P unchanged (R_L = V R_hat_L V^H,
+    a = V 1); the Butler rows reproduce the beamspace path (loading in beamspace)."""
+    rng = np.random.default_rng(25)
+    x = rng.standard_normal((2, 8, 14)) + 1j * rng.standard_normal((2, 8, 14))
+    L, Y, d = 6, 1, 0.05
+    e = O.capon_options(x, d, L, Y, fb=True)
+    s = O.capon_subspace(x, d, L, Y, np.eye(L), fb=True)
+    np.testing.assert_allclose(s["P"], e["P"], rtol=1e-12)
+    np.testing.assert_allclose(s["z"], e["z"], rtol=1e-10)
+    Q, _ = np.linalg.qr(rng.standard_normal((L, L)) + 1j * rng.standard_normal((L, L)))
+    u = O.capon_subspace(x, d, L, Y, Q.conj().T)
+    np.testing.assert_allclose(u["P"], O.capon(x, d, L, Y)["P"], rtol=1e-10)
+    Nb = 3
+    B = O.butler_matrix(Nb, L)
+    b = O.capon_subspace(x, d, L, Y, B)
+    r = O.capon_beamspace(x, d, L, Y, Nb)
+    np.testing.assert_allclose(b["P"], r["P"], rtol=1e-10)
+    np.testing.assert_allclose(b["z"], r["z"], rtol=1e-9)
+    np.testing.assert_allclose(b["w"], r["w_bs"], atol=1e-10)
