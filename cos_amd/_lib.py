@@ -15,7 +15,7 @@ EXPORTS = (
     "cos_capon_get_capon", "cos_capon_get_capon_p", "cos_buildr_get_r", "cos_solver_solve",
     "cos_capon_last_error", "cos_capon_plan_create", "cos_capon_plan_destroy", "cos_capon_run",
     "cos_capon_last_fallback_count", "cos_capon_beamspace_run", "cos_capon_backward",
-    "cos_capon_subspace_run",
+    "cos_capon_subspace_run", "cos_tdib_run", "cos_lca_run",
 )
 
 COS_PREC_FP32 = 0
@@ -76,6 +76,9 @@ def lib() -> C.CDLL:
     L.cos_capon_beamspace_run.argtypes = [vp, C.POINTER(CaponDesc), ip, vp, vp, vp, vp, vp, ip, vp]
     L.cos_capon_backward.argtypes = [vp, C.POINTER(CaponDesc), vp, vp, vp, vp, vp]
     L.cos_capon_subspace_run.argtypes = [vp, C.POINTER(CaponDesc), vp, ip, vp, vp, vp, vp, vp]
+    db = C.c_double
+    L.cos_tdib_run.argtypes = [vp, ip, ip, db, db, vp, vp, db, db, vp, vp, ip, ip, fl, vp, vp, vp]
+    L.cos_lca_run.argtypes = [vp, ip, ip, ip, vp, ip, ip, vp, vp, vp]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.cos_capon_last_error.restype = C.c_char_p

@@ -17,6 +17,7 @@
 #include "capon_kernels.h"
 #include "beamspace.h"
 #include "components.h"
+#include "aux_stages.h"
 
 namespace {
 
@@ -400,6 +401,56 @@ int cos_capon_subspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, con
                                 ds.off, ds.Yavg, ds.Nx, ds.Nz, desc->d, (float2*)w, (float2*)z, P, s)) != hipSuccess)
     return fail_hip(e, "subspace kernel");
   return 0;
+}
+
+int cos_tdib_run(const cos_cf32* mfdata, int n_h, int n_t, double t0, double dt, const double* x_r,
+                 const double* D, double c, double fc, const double* xarr, const double* yarr,
+                 int n_x, int n_y, float fill, cos_cf32* cube, cos_cf32* image, void* stream) {
+  g_last_error.clear();
+  if (!mfdata || !x_r || !D || !xarr || !yarr) return fail(hipErrorInvalidValue, "NULL input");
+  if (n_h < 1 || n_t < 1 || n_x < 1 || n_y < 1) return fail(hipErrorInvalidValue, "sizes must be >= 1");
+  if (!(dt > 0.0) || !(c > 0.0) || !std::isfinite(t0) || !std::isfinite(fc))
+    return fail(hipErrorInvalidValue, "need dt > 0, c > 0, finite t0 and fc");
+  if ((long)n_x * n_y > (1L << 31) - 1) return fail(hipErrorInvalidValue, "too many pixels for one call");
+  cosk::TdibLaunch a;
+  a.mf = (const float2*)mfdata;
+  a.n_h = n_h;
+  a.n_t = n_t;
+  a.t0 = t0;
+  a.dt = dt;
+  a.x_r = x_r;
+  a.D = D;
+  a.inv_c = 1.0 / c;
+  a.fc = fc;
+  a.xarr = xarr;
+  a.yarr = yarr;
+  a.n_x = n_x;
+  a.n_y = n_y;
+  a.fill = fill;
+  a.cube = (float2*)cube;
+  a.image = (float2*)image;
+  hipError_t e = cosk::launch_tdib(a, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : fail_hip(e, "tdib kernel");
+}
+
+int cos_lca_run(const cos_cf32* Z, int Nx, int Ny, int Nw, const float* wrf, int wy, int wx,
+                cos_cf32* img, int* win, void* stream) {
+  g_last_error.clear();
+  if (!Z || !wrf) return fail(hipErrorInvalidValue, "NULL input");
+  if (Nx < 1 || Ny < 1 || Nw < 1 || wy < 1 || wx < 1) return fail(hipErrorInvalidValue, "sizes must be >= 1");
+  if (wy % 2 == 0 || wx % 2 == 0) return fail(hipErrorInvalidValue, "the wrf filter must have odd sizes");
+  cosk::LcaLaunch a;
+  a.Z = (const float2*)Z;
+  a.Nx = Nx;
+  a.Ny = Ny;
+  a.Nw = Nw;
+  a.wrf = wrf;
+  a.wy = wy;
+  a.wx = wx;
+  a.img = (float2*)img;
+  a.win = win;
+  hipError_t e = cosk::launch_lca(a, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : fail_hip(e, "lca kernel");
 }
 
 int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, int Nb, const cos_cf32* x,

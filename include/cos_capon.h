@@ -156,6 +156,29 @@ int cos_capon_subspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, con
 int cos_capon_backward(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x,
                        const float* grad_P, cos_cf32* grad_x, float* grad_d, void* stream);
 
+/* The stages either side of the beamformer call.
+ *
+ * Delay stage: time-domain interpolation beamformer, replacing
+ * framework/beamformer/delay/tdibC.pyx:9-67 (tdib(mfdata, mtaxe, x_r, D, c, fc, xarr, yarr)).
+ * mfdata [n_h][n_t] matched-filtered channels on the uniform axis t0 + i*dt (the reference
+ * takes any sorted axis; this entry takes a uniform one); x_r, D [n_h] hydrophone positions;
+ * xarr, yarr [n_x][n_y] pixel coordinates; all device pointers.  Writes the delayed,
+ * carrier-mixed channels cube [n_x][n_y][n_h] (the Capon input layout) and/or the
+ * delay-and-sum image [n_y][n_x] = (1/n_h) sum over hydrophones (the reference's return
+ * value); either may be NULL.  Times outside the axis give `fill` (the reference: NaN). */
+int cos_tdib_run(const cos_cf32* mfdata, int n_h, int n_t, double t0, double dt, const double* x_r,
+                 const double* D, double c, double fc, const double* xarr, const double* yarr,
+                 int n_x, int n_y, float fill, cos_cf32* cube, cos_cf32* image, void* stream);
+
+/* Window selection: low-complexity adaptive beamformer, replacing
+ * framework/beamformer/lca/lcacx.pyx:27-217 (lca(Z, wrf)).  Z [Nx][Ny][Nw] images formed
+ * with Nw windows; wrf [wy][wx] neighbourhood weights (odd sizes), device pointers.
+ * img [Nx][Ny] gets Z at the window whose weighted neighbourhood power is lowest (first
+ * minimum), win [Nx][Ny] that window's index; pixels whose neighbourhood leaves the image
+ * get 0 in both.  Either output may be NULL. */
+int cos_lca_run(const cos_cf32* Z, int Nx, int Ny, int Nw, const float* wrf, int wy, int wx,
+                cos_cf32* img, int* win, void* stream);
+
 /* Number of pixels the last cos_capon_run sent to the FP64 fallback (synchronises the
  * plan's stream). */
 int cos_capon_last_fallback_count(cos_capon_plan* plan, int* count);
