@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparator")
     ap.add_argument("--workload", default="frame",
-                    choices=["frame", "aperture", "beamspace", "steering", "adjoint"],
+                    choices=["frame", "aperture", "beamspace", "steering", "adjoint", "tdib", "lca"],
                     help="frame: the headline 256x2048x64 L=32 frame (default); the others time the "
                          "expanding-aperture, sonar beamspace, steered and differentiable workloads")
     args = ap.parse_args()

@@ -7,6 +7,8 @@ beamspace  sonar beamspace, 512 look directions x 8192 ranges x 96 elements, L =
 steering   far-field ULA, 1024 range bins x 32 elements, L = 16, 128 steering angles
 adjoint    differentiable frames: 8 x (128 x 512 x 64) with per-element phase errors,
            loss = -mean log P, forward + adjoint
+tdib       the delay stage producing a 256 x 2048 x 64 cube from 64 x 32768 samples
+lca        window selection over 8 windows on a 256 x 2048 image
 
 Inputs are generated on the GPU with the counter-based generator (cos_amd.synth), so
 they are resident in HBM before timing starts.  The oracle is used only for the accuracy
@@ -176,4 +178,58 @@ def adjoint(args):
                   "Yavg": Y, "pixels": npx})
     line["forward_ms"] = round(ms_f, 3)
     line["backward_over_forward"] = round((ms - ms_f) / ms_f, 2)
+    return line
+
+
+def tdib(args):
+    """Delay stage producing the headline frame's cube: 256 beams x 2048 range pixels x 64
+    hydrophones from 64 x 32768 matched-filtered samples; cube and image written."""
+    import torch
+    from cos_amd import stages
+    n_h, n_t, n_x, n_y = 64, 32768, 256, 2048
+    g = torch.Generator(device="cuda").manual_seed(7)
+    mf = torch.complex(torch.randn(n_h, n_t, device="cuda", generator=g),
+                       torch.randn(n_h, n_t, device="cuda", generator=g))
+    mtaxe = 0.01 + np.arange(n_t) * 4.0e-6
+    x_r = torch.linspace(0.0, 1.5, n_h, dtype=torch.float64, device="cuda")
+    D = torch.full((n_h,), 0.3, dtype=torch.float64, device="cuda")
+    xarr = torch.linspace(-6.0, 6.0, n_x, dtype=torch.float64, device="cuda")[:, None].expand(n_x, n_y).contiguous()
+    yarr = torch.linspace(15.0, 90.0, n_y, dtype=torch.float64, device="cuda")[None, :].expand(n_x, n_y).contiguous()
+    step = lambda: stages.tdib_cube(mf, mtaxe, x_r, D, 1500.0, 1.0e5, xarr, yarr, fill=0.0)
+    for _ in range(args.warmup):
+        step()
+    ms = _timed(torch, step, args.steps)
+    npx = n_x * n_y
+    # bytes: cube written (8 n_h per pixel) + image (8) + coordinates read (16); channel
+    # samples are read from cache (each pixel reads 2 samples per hydrophone)
+    bpx = 8 * n_h + 8 + 16
+    gbs = bpx * npx / (ms * 1e-3) / 1e9
+    line = _base(args, "delay stage: pixels/sec producing the 256x2048x64 Capon cube", npx / (ms * 1e-3),
+                 "pixels/s", ms, {"workload": "TDIB delay stage", "n_x": n_x, "n_y": n_y, "n_h": n_h, "n_t": n_t},
+                 dtype="f64 geometry, f32 data")
+    line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_pixel": bpx}
+    return line
+
+
+def lca(args):
+    """Window selection over 8 windows on a 256 x 2048 image, 5 x 1 range filter."""
+    import torch
+    from cos_amd import stages
+    Nx, Ny, Nw = 256, 2048, 8
+    g = torch.Generator(device="cuda").manual_seed(8)
+    Z = torch.complex(torch.randn(Nx, Ny, Nw, device="cuda", generator=g),
+                      torch.randn(Nx, Ny, Nw, device="cuda", generator=g))
+    wrf = np.ones((5, 1), np.float32)
+    step = lambda: stages.lca(Z, wrf)
+    for _ in range(args.warmup):
+        step()
+    ms = _timed(torch, step, args.steps)
+    npx = Nx * Ny
+    bpx = 8 * Nw + 8 + 4
+    gbs = bpx * npx / (ms * 1e-3) / 1e9
+    line = _base(args, "LCA window selection: pixels/sec, 8 windows, 5x1 filter", npx / (ms * 1e-3), "pixels/s", ms,
+                 {"workload": "LCA window selection", "Nx": Nx, "Ny": Ny, "Nw": Nw, "filter": [5, 1]})
+    line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_pixel": bpx}
     return line
