@@ -123,6 +123,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     dload = 1.f;
   }
   const int K = M - L + 1;
+  // 64-lane groups (one pixel per wave) run only the first L steps of the loops over
+  // matrix rows; the identity padding up to LMAX is skipped instead of factorised
+  constexpr bool SKIP = G == 64;
+  const int Lw = SKIP ? __builtin_amdgcn_readfirstlane(L) : LMAX;
 
   // Shared rows: when the wave's NG pixels are consecutive in one beam with the same
   // aperture (every wave of a uniform frame whose row count is a multiple of NG), their
@@ -260,6 +264,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     }
     static_for<1, LMAX>([&](auto R_) {
     constexpr int R = decltype(R_)::value;
+      if (SKIP && R >= Lw) return;
       cplx<T> nv[CPL];
       static_for<0, CPL>([&](auto S_) {
     constexpr int S = decltype(S_)::value;
@@ -434,6 +439,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   });
   static_for<0, LMAX>([&](auto Kk_) {
     constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
+    if (SKIP && k >= Lw) return;
     // A pivot that is not positive (or NaN) is clamped to tiny; the flag pass detects
     // the clamp from the largest 1/U_kk.  The loading is added in the identity padding
     // too, where it changes nothing (its right-hand side is 0).
@@ -476,10 +482,12 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       cplx<T> ur[LMAX];
       static_for<k + 1 + CHOL_REG_ROWS, LMAX>([&](auto Ii_) {
         constexpr int i = decltype(Ii_)::value;
+        if (SKIP && i >= Lw) return;
         ur[i] = rb[i];
       });
       static_for<k + 1, LMAX>([&](auto Ii_) {
         constexpr int i = decltype(Ii_)::value;
+        if (SKIP && i >= Lw) return;
         const cplx<T> u = (i <= k + CHOL_REG_ROWS) ? bcastc<G, i % G>(A[slot<G, i / G, k>()]) : ur[i];
         static_for<0, CPL>([&](auto S_) {
           constexpr int S = decltype(S_)::value;
@@ -494,7 +502,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     T maxinv = T(0);
     static_for<0, LMAX>([&](auto Kk_) {
       constexpr int k = decltype(Kk_)::value;
-      maxinv = fmax(maxinv, invd[k]);
+      if (!SKIP || k < Lw) maxinv = fmax(maxinv, invd[k]);
     });
     notpd = !(maxinv < T(0.5) * rsqrt_t<T>(tiny_value<T>()));
   }
@@ -525,6 +533,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       });
       static_for<0, LMAX>([&](auto Kk_) {
         constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
+        if (SKIP && k >= Lw) return;
         cplx<T> v1 = bcastc<G, lk>(bs[sk]);
         const T inv = invd[k];
         v1.x *= inv;
@@ -546,7 +555,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     T accP = T(0);
     static_for<0, CPL>([&](auto S_) {
       constexpr int S = decltype(S_)::value;
-      cr[S] = Upk[cs[S] + ucol[S]];
+      cr[S] = cs[S] < Lw ? Upk[cs[S] + ucol[S]] : cplx<T>{T(0), T(0)};
       accP = fma(cr[S].x, cr[S].x, fma(cr[S].y, cr[S].y, accP));
     });
     accP = Gp::sum(accP);
@@ -565,6 +574,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       });
       static_for<LMAX - 1, 0, -1>([&](auto Cc_) {
         constexpr int jj = decltype(Cc_)::value, sj = jj / G, lj = jj % G;
+        if (SKIP && jj >= Lw) return;
         const T inv = invd[jj];
         cplx<T> yb = bcastc<G, lj>(cr[sj]);
         yb.x *= inv;
@@ -580,7 +590,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       cplx<T> y[CPL];
       static_for<0, CPL>([&](auto S_) {
         constexpr int S = decltype(S_)::value;
-        const T inv = invd[cs[S]];
+        const T inv = cs[S] < Lw ? invd[cs[S]] : T(0);
         y[S] = {cr[S].x * inv, cr[S].y * inv};
       });
       // w = y / (a^H y), a^H y = |v|^2;  z = w^H g
