@@ -17,7 +17,8 @@
 // snapshot row runs the sliding DFT across its K subarrays and accumulates that row's
 // Nb x Nb Gram and its subarray mean.  The rows are staged through LDS in windows of W
 // elements (each wave loads its 64 rows with coalesced loads, then every thread walks its
-// own row in the window), so the cube is read once with full cache lines.  Both results
+// own row in the window; the next window's loads are in flight meanwhile), so the cube is
+// read with few, wide requests.  Both results
 // go to LDS, so each snapshot is transformed once and shared by the 2Y+1 pixels whose
 // window contains it.  Phase 2: one thread per pixel sums its window's Grams, factors the
 // Nb x Nb system in registers and writes the outputs.
@@ -29,22 +30,32 @@ namespace cosk {
 template <int NB>
 struct BsTri { static constexpr int N = NB * (NB + 1) / 2; };
 
-constexpr int BS_W = 8;            // window width (elements) of the staged rows
+constexpr int BS_W = 16;           // window width (elements): one 128-byte line of a row
 constexpr int BS_WP = BS_W + 1;    // padded row stride in LDS (complex), spreads the banks
+constexpr int BS_LD = 64 / (64 / BS_W);   // loads per lane per window (64 rows x BS_W elements)
+constexpr unsigned BS_OOB = 0x80000000u;  // a buffer offset past any wave's records
 
-// dst[r][i] = x[row r of this wave][m0 + i] for i < BS_W (0 outside the row or the frame)
-__device__ __forceinline__ void bs_stage(float2* dst, const float2* xw, int nrows, int Mstride, int M, int m0,
-                                         int lane) {
+// Window m0 of this wave's rows into registers: p[rr] = x[row rr*(64/BS_W) + lane/BS_W]
+// [m0 + c], c = lane%BS_W, or 0 outside the row or the frame.  The loads go through a
+// buffer descriptor that covers exactly the wave's valid elements, so a row past the
+// frame, or a column outside [0, M) (given offset BS_OOB), reads as 0 with no branch and
+// no memory access; all BS_LD loads of a window are in flight together.
+__device__ __forceinline__ void bs_fetch(float2 (&p)[BS_LD], __amdgpu_buffer_rsrc_t rs, unsigned lanebase,
+                                         unsigned rowstep, int M, int m0, int c) {
+  const unsigned vb = (unsigned)(m0 + c) < (unsigned)M ? lanebase + (unsigned)m0 * 8u : BS_OOB;
 #pragma unroll
-  for (int rr = 0; rr < 64 / (64 / BS_W); ++rr) {
-    const int r = rr * (64 / BS_W) + lane / BS_W, i = lane % BS_W, m = m0 + i;
-    float2 v = make_float2(0.f, 0.f);
-    if (r < nrows && m >= 0 && m < M) v = xw[(size_t)r * Mstride + m];
-    dst[r * BS_WP + i] = v;
-  }
+  for (int rr = 0; rr < BS_LD; ++rr)
+    p[rr] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, vb + (unsigned)rr * rowstep, 0, 0));
 }
 
-template <int NB, int RB>
+// dst[r][i] (row stride BS_WP) from the registers of bs_fetch
+__device__ __forceinline__ void bs_put(float2* dst, const float2 (&p)[BS_LD], int lane) {
+#pragma unroll
+  for (int rr = 0; rr < BS_LD; ++rr) dst[(rr * (64 / BS_W) + lane / BS_W) * BS_WP + lane % BS_W] = p[rr];
+}
+
+// ZM: the subarray means (needed only for the amplitude z) are accumulated
+template <int NB, int RB, bool ZM>
 __global__ __launch_bounds__(RB) void beamspace_capon_kernel(const BeamspaceLaunch a) {
   constexpr int NT = BsTri<NB>::N;
   constexpr int NWAVE = RB / 64;
@@ -68,74 +79,96 @@ __global__ __launch_bounds__(RB) void beamspace_capon_kernel(const BeamspaceLaun
 #pragma unroll
   for (int b = 0; b < NB; ++b) sm[b] = make_float2(0.f, 0.f);
   {
-    const int wv = r / 64, lane = r % 64;
+    const int wv = __builtin_amdgcn_readfirstlane(r / 64), lane = r % 64;
     const int tw0 = t0 + wv * 64;                        // first snapshot row of this wave
     const int nrows = min(64, a.Nz - tw0);               // rows of this wave inside the frame
-    const float2* xw = a.x + ((size_t)ix * a.Nz + tw0) * a.Mstride + a.off;
-    float2* tail = smem + wv * 2 * 64 * BS_WP;
-    float2* head = tail + 64 * BS_WP;
-    const float2* mine_t = tail + lane * BS_WP;
+    // descriptor over rows [tw0, tw0 + nrows), elements [0, M) of the last one; a wave
+    // past the frame's end (nrows <= 0) has no records and reads zeros
+    const float2* xw = a.x + ((size_t)ix * a.Nz + min(tw0, a.Nz - 1)) * a.Mstride + a.off;
+    const unsigned nrec = nrows > 0 ? (unsigned)(((nrows - 1) * a.Mstride + M) * 8) : 0u;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)xw, (short)0, (int)nrec, 0x00020000);
+    const unsigned lanebase = (unsigned)((lane / BS_W) * a.Mstride + lane % BS_W) * 8u;
+    const unsigned rowstep = (unsigned)((64 / BS_W) * a.Mstride) * 8u;
+    const int col = lane % BS_W;
+    float2* tail_buf = smem + wv * 2 * 64 * BS_WP;
+    float2* head = tail_buf + 64 * BS_WP;
+    const float2* mine_t = tail_buf + lane * BS_WP;
     const float2* mine_h = head + lane * BS_WP;
     float2 S[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) S[b] = make_float2(0.f, 0.f);
-    // S_0[b] = sum_{l<L} x[l] tw[b][l]
-    for (int m0 = 0; m0 < L; m0 += BS_W) {
-      bs_stage(tail, xw, nrows, a.Mstride, M, m0, lane);
-      for (int i = 0; i < BS_W && m0 + i < L; ++i) {
-        const float2 xv = mine_t[i];
-        const int l = m0 + i;
+    // Windows go global -> registers one window ahead of the LDS window being walked; the
+    // LDS buffers are private to the wave, whose LDS accesses complete in program order.
+    // One pass over the row: S <- (S + x[l] - x[l-L]) omega for l = 0 .. M-1, x[<0] = 0.
+    // From an all-zero window the first L steps build S_0[b] = sum_{l<L} x[l] exp(-j 2 pi
+    // beta l / L) (omega^L = 1; the FP32 error is that of the direct sum, about 2e-6 of
+    // |x|), and step l >= L-1 leaves S_k, k = l - L + 1, the sliding DFT
+    // S_k = (S_{k-1} + x[k+L-1] - x[k-1]) omega.  The head windows start on multiples of
+    // BS_W, so each cache line is fetched once; the tail stream re-reads the head L
+    // elements behind.
+    float2 om[NB];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const float2 tw = a.tw[b * L + l];
-          S[b].x = fmaf(xv.x, tw.x, fmaf(-xv.y, tw.y, S[b].x));
-          S[b].y = fmaf(xv.x, tw.y, fmaf(xv.y, tw.x, S[b].y));
-        }
-      }
-    }
+    for (int b = 0; b < NB; ++b) om[b] = a.omega[b];
+    // The L^-1/2 of B is left out of the loop: the Grams and means are scaled once, in
+    // phase 2.  A diagonal Gram entry is real, so only its real part is accumulated.
     auto accumulate = [&]() {
-      float2 v[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        v[b] = make_float2(S[b].x * rsL, S[b].y * rsL);
-        sm[b].x += v[b].x;
-        sm[b].y += v[b].y;
+        if (!ZM) break;
+        sm[b].x += S[b].x;
+        sm[b].y += S[b].y;
       }
       int e = 0;
 #pragma unroll
       for (int b1 = 0; b1 < NB; ++b1) {
+        G[e].x = fmaf(S[b1].x, S[b1].x, fmaf(S[b1].y, S[b1].y, G[e].x));
+        ++e;
 #pragma unroll
-        for (int b2 = b1; b2 < NB; ++b2, ++e) {   // G[b1][b2] += v[b1] conj(v[b2])
-          G[e].x = fmaf(v[b1].x, v[b2].x, fmaf(v[b1].y, v[b2].y, G[e].x));
-          G[e].y = fmaf(v[b1].y, v[b2].x, fmaf(-v[b1].x, v[b2].y, G[e].y));
+        for (int b2 = b1 + 1; b2 < NB; ++b2, ++e) {   // G[b1][b2] += v[b1] conj(v[b2])
+          G[e].x = fmaf(S[b1].x, S[b2].x, fmaf(S[b1].y, S[b2].y, G[e].x));
+          G[e].y = fmaf(S[b1].y, S[b2].x, fmaf(-S[b1].x, S[b2].y, G[e].y));
         }
       }
     };
-    accumulate();
-    // S_k = (S_{k-1} + x[k+L-1] - x[k-1]) omega, k = 1 .. K-1: windows of the outgoing
-    // (tail) and incoming (head) elements
-    for (int k0 = 1; k0 < K; k0 += BS_W) {
-      bs_stage(tail, xw, nrows, a.Mstride, M, k0 - 1, lane);
-      bs_stage(head, xw, nrows, a.Mstride, M, k0 + L - 1, lane);
-      for (int i = 0; i < BS_W && k0 + i < K; ++i) {
-        const float2 xin = mine_h[i], xout = mine_t[i];
-        const float2 dlt = make_float2(xin.x - xout.x, xin.y - xout.y);
+    auto slide = [&](float2 dlt) {
+      S[0] = make_float2(S[0].x + dlt.x, S[0].y + dlt.y);   // beta(0) = 0: omega = 1
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const float2 u = make_float2(S[b].x + dlt.x, S[b].y + dlt.y);
-          const float2 om = a.omega[b];
-          S[b] = make_float2(fmaf(u.x, om.x, -u.y * om.y), fmaf(u.x, om.y, u.y * om.x));
+      for (int b = 1; b < NB; ++b) {
+        const float2 u = make_float2(S[b].x + dlt.x, S[b].y + dlt.y);
+        S[b] = make_float2(fmaf(u.x, om[b].x, -u.y * om[b].y), fmaf(u.x, om[b].y, u.y * om[b].x));
+      }
+    };
+    float2 pt[BS_LD], ph[BS_LD];
+    bs_fetch(ph, rs, lanebase, rowstep, M, 0, col);
+    if (BS_W >= L) bs_fetch(pt, rs, lanebase, rowstep, M, -L, col);
+    for (int p = 0; p < M; p += BS_W) {
+      // the tail window [p-L, p-L+BS_W) is walked once p + BS_W >= L (zeros below 0); the
+      // steps of the earlier windows are all below L - 1 and accumulate nothing
+      const bool tail = p + BS_W >= L;
+      bs_put(head, ph, lane);
+      if (tail) bs_put(tail_buf, pt, lane);
+      if (p + BS_W < M) {
+        bs_fetch(ph, rs, lanebase, rowstep, M, p + BS_W, col);
+        if (p + 2 * BS_W >= L) bs_fetch(pt, rs, lanebase, rowstep, M, p + BS_W - L, col);
+      }
+      if (tail) {
+        for (int i = 0; i < BS_W && p + i < M; ++i) {
+          const float2 xin = mine_h[i], xout = mine_t[i];
+          slide(make_float2(xin.x - xout.x, xin.y - xout.y));
+          if (p + i >= L - 1) accumulate();
         }
-        accumulate();
+      } else {
+        for (int i = 0; i < BS_W; ++i) slide(mine_h[i]);
       }
     }
   }
   __syncthreads();   // the windows are dead; the same LDS holds the Grams
 #pragma unroll
   for (int e = 0; e < NT; ++e) gram[r][e] = G[e];
-  const float invK = 1.f / (float)K;
+  const float invK = rsL / (float)K;                  // B's L^-1/2 and the mean's 1/K
 #pragma unroll
-  for (int b = 0; b < NB; ++b) smean[r][b] = make_float2(sm[b].x * invK, sm[b].y * invK);
+  for (int b = 0; b < NB; ++b)
+    if (ZM) smean[r][b] = make_float2(sm[b].x * invK, sm[b].y * invK);
   __syncthreads();
 
   // ---------------- phase 2: one pixel per thread ----------------
@@ -152,7 +185,7 @@ __global__ __launch_bounds__(RB) void beamspace_capon_kernel(const BeamspaceLaun
       R[e].y += gv.y;
     }
   }
-  const float nrm = 1.f / ((float)K * (float)TW);
+  const float nrm = 1.f / ((float)K * (float)TW * (float)L);   // with B's (L^-1/2)^2
   float tr = 0.f;
   {
     int e = 0;
@@ -232,13 +265,15 @@ __global__ __launch_bounds__(RB) void beamspace_capon_kernel(const BeamspaceLaun
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const float2 wb = make_float2(y[b].x * iq, y[b].y * iq);
-    const float2 s = smean[r + Y][b];
-    zz.x += wb.x * s.x + wb.y * s.y;                    // conj(w) s
-    zz.y += wb.x * s.y - wb.y * s.x;
+    if (ZM) {
+      const float2 s = smean[r + Y][b];
+      zz.x += wb.x * s.x + wb.y * s.y;                  // conj(w) s
+      zz.y += wb.x * s.y - wb.y * s.x;
+    }
     y[b] = wb;
     if (a.wbs) a.wbs[p * NB + b] = wb;
   }
-  if (a.z) a.z[p] = zz;
+  if (ZM && a.z) a.z[p] = zz;
   if (a.w) {
     for (int l = 0; l < L; ++l) {                       // w = B^H w_bs
       float2 acc = make_float2(0.f, 0.f);
@@ -260,7 +295,10 @@ static hipError_t launch_bs(const BeamspaceLaunch& a, hipStream_t s) {
   if (TP < 1) return hipErrorInvalidValue;
   const int Nzp = a.Nz - 2 * a.Y;
   dim3 grid((Nzp + TP - 1) / TP, a.Nx);
-  hipLaunchKernelGGL((beamspace_capon_kernel<NB, RB>), grid, dim3(RB), 0, s, a);
+  if (a.z)
+    hipLaunchKernelGGL((beamspace_capon_kernel<NB, RB, true>), grid, dim3(RB), 0, s, a);
+  else
+    hipLaunchKernelGGL((beamspace_capon_kernel<NB, RB, false>), grid, dim3(RB), 0, s, a);
   return hipGetLastError();
 }
 

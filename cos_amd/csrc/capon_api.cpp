@@ -466,6 +466,9 @@ int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, in
   if (Nb < 1 || Nb > 8) return fail(hipErrorInvalidValue, "beamspace path supports 1 <= Nb <= 8");
   if (ds.off < 0 || ds.off + ds.M > ds.Mstride) return fail(hipErrorInvalidValue, "off + M > Mstride");
   if (2 * ds.Yavg > 120) return fail(hipErrorInvalidValue, "beamspace path supports Yavg <= 60");
+  // the kernel addresses a wave's 64 rows with 32-bit buffer offsets below 2^31
+  if ((long long)ds.Mstride * 64 * 8 >= (1LL << 31))
+    return fail(hipErrorInvalidValue, "beamspace path supports Mstride < 4194304");
   if (w && wstride < ds.L) return fail(hipErrorInvalidValue, "wstride < L");
   hipError_t e;
   if (plan->bs_Nb != Nb || plan->bs_L != ds.L) {

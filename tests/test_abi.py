@@ -66,6 +66,10 @@ def test_plan_run_rejects_bad_descriptor():
     ds = CaponDesc(Nx=1, Nz=8, Mstride=8, M=8, L=4, off=2, Yavg=1, d=0.1)
     code, text = _err(L.cos_capon_run(h, C.byref(ds), 16, None, None, None, 0, None, None, None))
     assert code != 0 and "off + M > Mstride" in text
+    # the beamspace kernel's 32-bit row offsets bound the element stride
+    ds = CaponDesc(Nx=1, Nz=8, Mstride=1 << 22, M=8, L=4, off=0, Yavg=1, d=0.1)
+    code, text = _err(L.cos_capon_beamspace_run(h, C.byref(ds), 3, 16, None, None, None, None, 0, None))
+    assert code != 0 and "Mstride < 4194304" in text
     assert L.cos_capon_plan_destroy(h) == 0
 
 

@@ -354,13 +354,16 @@ def test_graph_replay_matches_eager():
 
 # ------------------------------------------------------------------ sliding-DFT beamspace
 
-@pytest.mark.parametrize("M,L,Y,Nb,d", [(12, 6, 1, 2, 0.2), (32, 16, 2, 5, 0.05), (20, 9, 0, 1, 0.1),
-                                        (40, 20, 3, 8, 0.01), (16, 8, 1, 4, 0.1)])
-def test_beamspace_sliding_matches_oracle(M, L, Y, Nb, d):
+@pytest.mark.parametrize("M,L,Y,Nb,d,Nz", [(12, 6, 1, 2, 0.2, 40), (32, 16, 2, 5, 0.05, 40), (20, 9, 0, 1, 0.1, 40),
+                                           (40, 20, 3, 8, 0.01, 40), (16, 8, 1, 4, 0.1, 40),
+                                           # K = 1 (no sliding step); several blocks, the last one
+                                           # with a wave wholly past the frame, odd L and K
+                                           (8, 8, 1, 3, 0.1, 40), (50, 23, 2, 5, 0.02, 300)])
+def test_beamspace_sliding_matches_oracle(M, L, Y, Nb, d, Nz):
     import torch
     from cos_amd import CaponPlan
-    say("beamspace sliding", M, L, Y, Nb, d)
-    x = uniform_cube(300 + Nb + M, 3, 40, M)
+    say("beamspace sliding", M, L, Y, Nb, d, Nz)
+    x = uniform_cube(300 + Nb + M, 3, Nz, M)
     o = CaponPlan().run_beamspace(torch.from_numpy(x).cuda(), d, L, Y, Nb, want=("P", "z", "w_bs", "w"))
     torch.cuda.synchronize()
     ref = O.capon_beamspace(x.astype(np.complex128), d, L, Y, Nb)
@@ -375,12 +378,13 @@ def test_beamspace_sliding_matches_oracle(M, L, Y, Nb, d):
 
 
 def test_beamspace_sonar_slice():
-    """The 96-element, L = 48, Nb = 5, Yavg = 4 sonar geometry on 8 look directions."""
+    """The 96-element, L = 48, Nb = 5, Yavg = 4 sonar geometry on 8 look directions, power
+    only (the kernel variant that accumulates no subarray means)."""
     import torch
     from cos_amd import CaponPlan, synth
     say("beamspace sonar slice", 8, 1024, 96, 48, 5)
     x = synth.config4(Nx=8, Nz=1024, M=96, seed=4)
-    o = CaponPlan().run_beamspace(torch.from_numpy(x).cuda(), 0.01, 48, 4, 5)
+    o = CaponPlan().run_beamspace(torch.from_numpy(x).cuda(), 0.01, 48, 4, 5, want=("P",))
     torch.cuda.synchronize()
     P = o["P"].cpu().numpy()
     ix, j = O.all_pixels(8, 1024, 4)
