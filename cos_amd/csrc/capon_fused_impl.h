@@ -661,50 +661,83 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         const T coeff = -gPp / (norm * accP * accP);            // dl/dq
         const T cload = (T)dload * yn / (T)L;
         if (active && g == 0 && a.gd) a.gd[p] = (float)(coeff * (-trS * yn / (T)L));
-        // the window rows again (the group's region held U), then beta after them
+        // the window rows again (the group's region held U), then beta after them with
+        // LMAX zeros in front and zeros for K <= k < KS*G, so the convolution below
+        // needs no bounds tests
+        constexpr int KS = (BWD + G - 1) / G;
         float2* xr0 = xs_all + q * GREG;
-        cplx<T>* beta = reinterpret_cast<cplx<T>*>(xr0 + TW * MP);
+        cplx<T>* beta = reinterpret_cast<cplx<T>*>(xr0 + TW * MP) + LMAX;
         const float2* srcw = a.x + ((size_t)ix * a.Nz + j) * a.Mstride + off;
         for (int t = 0; t < TW; ++t)
           for (int m = g; m < MP; m += G)
             xr0[t * MP + m] = (m < M) ? srcw[(size_t)t * a.Mstride + m] : make_float2(0.f, 0.f);
+        for (int i = g; i < LMAX; i += G) beta[i - LMAX] = {T(0), T(0)};
         __syncthreads();
-        constexpr int KS = (BWD + G - 1) / G;
+        // lane slots k = g + G*S that hold some k < K: KA = ceil(K/G), uniform
+        const int KA = (K + G - 1) / G;
+        cplx<T> yc[CPL];
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          yc[S] = {y[S].x, -y[S].y};
+        });
         for (int t = 0; t < TW; ++t) {
           const float2* xt = xr0 + t * MP;
-          cplx<T> bacc[KS];
-          static_for<0, KS>([&](auto S_) {
-            constexpr int S = decltype(S_)::value;
-            bacc[S] = {T(0), T(0)};
-          });
-          static_for<0, LMAX>([&](auto l_) {
-            constexpr int l = decltype(l_)::value;
-            const cplx<T> yb = bcastc<G, l % G>(y[l / G]);
+          // beta_{t,k} = y^H u_{t,k} over the KA live slots.  For k < K every read is in
+          // row t (k + l <= M - 1 where y_l != 0); the lanes with k >= K read the row's
+          // zero padding, the next row or the zeros in front of beta (at most G - 7
+          // elements past the last row, inside the group's region), and are not kept
+          static_for<1, KS + 1>([&](auto KA_) {
+            constexpr int NA = decltype(KA_)::value;
+            if (KA != NA) return;
+            cplx<T> nb[NA];   // -beta
+            static_for<0, NA>([&](auto S_) {
+              constexpr int S = decltype(S_)::value;
+              nb[S] = {T(0), T(0)};
+            });
+            static_for<0, LMAX>([&](auto l_) {
+              constexpr int l = decltype(l_)::value;
+              static_for<0, (NA + 1) / 2>([&](auto P_) {
+                constexpr int S = 2 * decltype(P_)::value;
+                if constexpr (S + 1 < NA)
+                  bcast_sub_conj_mul2<G, l % G>(nb[S], nb[S + 1], y[l / G], ldx<T>(xt + g + G * S + l),
+                                            ldx<T>(xt + g + G * (S + 1) + l));
+                else
+                  bcast_sub_conj_mul<G, l % G>(nb[S], y[l / G], ldx<T>(xt + g + G * S + l));
+              });
+            });
             static_for<0, KS>([&](auto S_) {
               constexpr int S = decltype(S_)::value;
               const int k = g + G * S;
-              if (k < K) add_conj_mul(bacc[S], yb, ldx<T>(xt + k + l));
+              cplx<T> v = {T(0), T(0)};
+              if constexpr (S < NA) v = {-nb[S].x, -nb[S].y};
+              beta[k] = k < K ? v : cplx<T>{T(0), T(0)};
             });
           });
-          static_for<0, KS>([&](auto S_) {
-            constexpr int S = decltype(S_)::value;
-            const int k = g + G * S;
-            if (k < K) beta[k] = bacc[S];
-          });
           __syncthreads();
-          cplx<T> gacc[KS];
+          // gacc_m = sum_l y_l beta_{t,m-l}, accumulated negated as -conj(conj(y_l)) beta;
+          // slot S holds m < G(S+1), so l > G(S+1)-1 meets only the zero prefix and is
+          // skipped at compile time
+          cplx<T> ng[KS];
           static_for<0, KS>([&](auto S_) {
             constexpr int S = decltype(S_)::value;
-            gacc[S] = {T(0), T(0)};
+            ng[S] = {T(0), T(0)};
           });
           static_for<0, LMAX>([&](auto l_) {
             constexpr int l = decltype(l_)::value;
-            const cplx<T> yb = bcastc<G, l % G>(y[l / G]);
-            static_for<0, KS>([&](auto S_) {
+            constexpr int S0 = l < G ? 0 : (l - G + 1 + G - 1) / G;   // first slot with l <= G*S + G - 1
+            static_for<S0, KS, 2>([&](auto S_) {
               constexpr int S = decltype(S_)::value;
-              const int m = g + G * S, kk = m - l;
-              if (m < M && kk >= 0 && kk < K) add_mul(gacc[S], yb, beta[kk]);
+              if constexpr (S + 1 < KS)
+                bcast_sub_conj_mul2<G, l % G>(ng[S], ng[S + 1], yc[l / G], beta[g + G * S - l],
+                                          beta[g + G * (S + 1) - l]);
+              else
+                bcast_sub_conj_mul<G, l % G>(ng[S], yc[l / G], beta[g + G * S - l]);
             });
+          });
+          cplx<T> gacc[KS];
+          static_for<0, KS>([&](auto S_) {
+            constexpr int S = decltype(S_)::value;
+            gacc[S] = {-ng[S].x, -ng[S].y};
           });
           if (active && valid) {
             static_for<0, KS>([&](auto S_) {
@@ -767,9 +800,10 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   const int upk = (LMAX * (LMAX + 1) / 2 + 2 + 2 * LMAX) * (int)(sizeof(cplx<T>) / sizeof(float2)) +
                   LMAX * (int)sizeof(T) / (int)sizeof(float2);
   int GREG = TW * MP > upk ? TW * MP : upk;
-  if (BWD) {   // window rows plus beta[K <= BWD] for the adjoint epilogue
+  if (BWD) {   // window rows plus the zero-padded beta of the adjoint epilogue
     if (Mmax > BWD) return hipErrorInvalidValue;
-    const int need = TW * MP + BWD * (int)(sizeof(cplx<T>) / sizeof(float2));
+    constexpr int KSG = (BWD + G - 1) / G * G;   // beta slots, after LMAX zeros
+    const int need = TW * MP + (LMAX + KSG) * (int)(sizeof(cplx<T>) / sizeof(float2));
     GREG = GREG > need ? GREG : need;
   }
   const size_t lds = (size_t)NG * GREG * sizeof(float2);

@@ -188,6 +188,31 @@ __device__ __forceinline__ void sub_conj_bcast_mul2(cplx<float>& a, cplx<float>&
 #undef COS_BC1
 #undef COS_FMAC_BC
 
+// a -= conj(u_L) v (and b -= conj(u_L) w), u_L = u in lane L of the lane's G-lane group:
+// the fused DPP form (with its wait states) for FP32 16-lane groups, an explicit
+// broadcast otherwise.  Nothing broadcast stays live in registers between calls.
+template <int G, int L, typename T>
+__device__ __forceinline__ void bcast_sub_conj_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
+  if constexpr (G == 16 && std::is_same<T, float>::value) {
+    sub_conj_bcast_mul<L, true>(a, u, v);
+  } else {
+    const cplx<T> ub = bcastc<G, L>(u);
+    a.x = fma(-ub.x, v.x, a.x);
+    a.x = fma(-ub.y, v.y, a.x);
+    a.y = fma(-ub.x, v.y, a.y);
+    a.y = fma(ub.y, v.x, a.y);
+  }
+}
+template <int G, int L, typename T>
+__device__ __forceinline__ void bcast_sub_conj_mul2(cplx<T>& a, cplx<T>& b, cplx<T> u, cplx<T> v, cplx<T> w) {
+  if constexpr (G == 16 && std::is_same<T, float>::value) {
+    sub_conj_bcast_mul2<L, true>(a, b, u, v, w);
+  } else {
+    bcast_sub_conj_mul<G, L>(a, u, v);
+    bcast_sub_conj_mul<G, L>(b, u, w);
+  }
+}
+
 template <typename T> __device__ __forceinline__ T rsqrt_t(T v);
 // v_rsq_f32 (1 ulp); the FP32 path's accuracy budget is set by the covariance sums.
 template <> __device__ __forceinline__ float rsqrt_t<float>(float v) { return __builtin_amdgcn_rsqf(v); }
