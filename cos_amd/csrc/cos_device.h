@@ -196,11 +196,7 @@ __device__ __forceinline__ void bcast_sub_conj_mul(cplx<T>& a, cplx<T> u, cplx<T
   if constexpr (G == 16 && std::is_same<T, float>::value) {
     sub_conj_bcast_mul<L, true>(a, u, v);
   } else {
-    const cplx<T> ub = bcastc<G, L>(u);
-    a.x = fma(-ub.x, v.x, a.x);
-    a.x = fma(-ub.y, v.y, a.x);
-    a.y = fma(-ub.x, v.y, a.y);
-    a.y = fma(ub.y, v.x, a.y);
+    sub_conj_mul(a, bcastc<G, L>(u), v);
   }
 }
 template <int G, int L, typename T>
