@@ -783,8 +783,14 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
     for (int gid0 = blockIdx.x * NGk; gid0 < npix; gid0 += gridDim.x * NGk)
       capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
   } else {
-    // one group of pixels per block (a loop here costs the FP32 kernel its registers)
-    const int gid0 = blockIdx.x * NGk;
+    // One group of pixels per block (a loop here costs the FP32 kernel its registers).
+    // Workgroups are dealt to the 8 XCDs round-robin, and each XCD has its own L2;
+    // consecutive groups share window rows, so the first 8*(B/8) blocks are renumbered
+    // to give each XCD a contiguous run of groups (a bijection; the tail keeps its order).
+    constexpr unsigned NXCD = 8;
+    const unsigned B = gridDim.x, B8 = B / NXCD * NXCD, b = blockIdx.x;
+    const unsigned lb = b < B8 ? (b % NXCD) * (B8 / NXCD) + b / NXCD : b;
+    const int gid0 = (int)lb * NGk;
     if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
   }
 }
