@@ -220,7 +220,7 @@ def lca(args):
     g = torch.Generator(device="cuda").manual_seed(8)
     Z = torch.complex(torch.randn(Nx, Ny, Nw, device="cuda", generator=g),
                       torch.randn(Nx, Ny, Nw, device="cuda", generator=g))
-    wrf = np.ones((5, 1), np.float32)
+    wrf = torch.ones((5, 1), dtype=torch.float32, device="cuda")
     step = lambda: stages.lca(Z, wrf)
     for _ in range(args.warmup):
         step()

@@ -85,7 +85,9 @@ def lca(Z, wrf, *, stream=None):
     Returns (img [Nx][Ny] complex64, win [Nx][Ny] int32) CUDA tensors."""
     import torch
     z = _dev(Z, torch.complex64)
-    w = _dev(np.atleast_2d(np.asarray(wrf.cpu() if hasattr(wrf, "cpu") else wrf, dtype=np.float32)), torch.float32)
+    w = _dev(wrf, torch.float32)          # a CUDA tensor is used where it is
+    if w.dim() < 2:                       # as numpy.atleast_2d: a vector is one row
+        w = w.reshape(1, -1).contiguous()
     Nx, Ny, Nw = z.shape
     wy, wx = w.shape
     img = torch.empty((Nx, Ny), dtype=torch.complex64, device="cuda")

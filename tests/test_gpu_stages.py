@@ -66,15 +66,17 @@ def test_tdib_feeds_capon():
     assert (np.abs(P - ref["P"]) / ref["P"]).max() <= 1e-6
 
 
-@pytest.mark.parametrize("wy,wx", [(1, 1), (5, 1), (3, 3)])
-def test_lca_matches_oracle(wy, wx):
+@pytest.mark.parametrize("wy,wx,Nw", [(1, 1, 6), (5, 1, 6), (3, 3, 6), (3, 5, 16),
+                                      # past the tiled kernel's limits: Nw > 16, Ym > 32
+                                      (5, 3, 20), (67, 1, 4)])
+def test_lca_matches_oracle(wy, wx, Nw):
     """Selected windows equal the oracle's wherever its best and second-best sums differ by
     more than 1e-5 relative (FP32 sums on the GPU); the selected values are then Z at that
     window, bit for bit; edges are zero."""
     from cos_amd import stages
-    say("lca", wy, wx)
-    rng = np.random.default_rng(wy * 10 + wx)
-    Nx, Ny, Nw = 24, 400, 6
+    say("lca", wy, wx, Nw)
+    rng = np.random.default_rng(wy * 10 + wx + Nw)
+    Nx, Ny = 24, 400
     Z = (rng.standard_normal((Nx, Ny, Nw)) + 1j * rng.standard_normal((Nx, Ny, Nw))).astype(np.complex64)
     wrf = rng.uniform(0.5, 1.0, (wy, wx)).astype(np.float32)
     img, win = stages.lca(Z, wrf)
@@ -92,3 +94,19 @@ def test_lca_matches_oracle(wy, wx):
     edge[Xm:Nx - Xm, Ym:Ny - Ym] = False
     np.testing.assert_array_equal(img[~edge], Z[xs, ys, win][~edge])
     assert (img[edge] == 0).all() and (win[edge] == 0).all()
+
+
+def test_lca_weights_as_device_vector():
+    """A 1-D weight vector (a CUDA tensor, used in place) is one filter row, as
+    numpy.atleast_2d makes it in the reference's call."""
+    import torch
+    from cos_amd import stages
+    say("lca device weights", 1, 3)
+    rng = np.random.default_rng(5)
+    Z = (rng.standard_normal((12, 300, 5)) + 1j * rng.standard_normal((12, 300, 5))).astype(np.complex64)
+    w = torch.tensor([0.5, 1.0, 0.5], device="cuda")
+    img, win = stages.lca(torch.from_numpy(Z).cuda(), w)
+    _, rwin, s = S.lca(Z.astype(np.complex128), np.array([[0.5, 1.0, 0.5]]))
+    srt = np.sort(s, axis=-1)
+    clear = (srt[..., 1] - srt[..., 0]) > 1e-5 * srt[..., 1]
+    np.testing.assert_array_equal(win.cpu().numpy()[1:-1, :][clear], rwin[1:-1, :][clear])
