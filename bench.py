@@ -45,6 +45,30 @@ def measured_traffic():
         return None
 
 
+def end_to_end(x, calls=3):
+    """The drop-in call with host buffers (cos_capon_get_capon_p: the library copies the
+    cube in and z, w, P out over PCIe, R not requested), wall time per call after one
+    warm-up call.  Reported beside the resident-cube figure, never as `value`."""
+    import ctypes as C
+    from cos_amd._lib import lib
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    xc = np.ascontiguousarray(x, dtype=np.complex64)
+    z = np.empty((NX, NZ - 2 * Y), np.complex64)
+    w = np.empty((NX, NZ - 2 * Y, L), np.complex64)
+    P = np.empty((NX, NZ - 2 * Y), np.float32)
+    call = lambda: lib().cos_capon_get_capon_p(ptr(z), ptr(w), None, ptr(P), ptr(xc), D, L, Y, M, NX, NZ, 0)
+    if call() != 0:
+        return None
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        call()
+    ms = (time.perf_counter() - t0) / calls * 1e3
+    host_bytes = xc.nbytes + z.nbytes + w.nbytes + P.nbytes
+    return {"ms_per_frame": round(ms, 3), "pixels_per_s": NPX / (ms * 1e-3),
+            "host_bytes": host_bytes, "calls": calls,
+            "path": "cos_capon_get_capon_p, pageable host buffers: cube in; z, w, P out; R not requested"}
+
+
 def cpu_baseline(x, budget_s=12.0):
     """The oracle's C FP64 restatement of the reference CPU path, all host cores, on a
     bounded sample of whole beams; rate scaled to pixels/s."""
@@ -172,6 +196,7 @@ def main():
         "input_gen_s": round(gen_s, 2),
     }
     line.update(res)
+    line["end_to_end"] = end_to_end(x_np)
     if not args.no_cpu:
         cb = cpu_baseline(x_np)
         line["cpu_baseline"] = cb
