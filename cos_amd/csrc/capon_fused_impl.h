@@ -26,11 +26,12 @@
 // The FP64 path ("plain" per-pixel kernel, also the fallback for ill-conditioned
 // pixels) uses direct sums for every element.
 //
-// Cholesky: right-looking U^H U on the upper triangle.  Each finished row k of U is
-// published in LDS and read back by the whole group (broadcast reads), so the trailing
-// update is FMAs only; the same row is stored packed for the back solve, and the
-// forward solve U^H v = a runs inside the factorisation (a as an extra column), so the
-// registers of A die row by row.  a^H S_L^-1 a = |v|^2.  The back solve U y = v is
+// Cholesky: right-looking U^H U on the upper triangle.  The multiplier U[k][i] of the
+// trailing update reaches the group by a broadcast: inside the FMAs (a DPP source) for
+// FP32 16-lane groups, by v_readlane for 64-lane groups, from a row published in LDS
+// for most rows of the FP64 16-lane groups.  Each finished row of U is stored packed in
+// LDS for the back solve, and the forward solve U^H v = a runs inside the factorisation
+// (a as an extra column), so the registers of A die row by row.  a^H S_L^-1 a = |v|^2.  The back solve U y = v is
 // column-oriented: one broadcast of y_j per column, U[i][j] read from LDS.
 #pragma once
 #include "cos_device.h"
@@ -48,8 +49,10 @@ __device__ __forceinline__ cplx<T> ldx(const float2* p) {
 // so K = M - L + 1) its epilogue handles with per-lane k and m slots (64 or 128).
 
 // Rows of each Cholesky step whose multiplier U[k][i] is broadcast from registers
-// rather than read back from LDS.
-constexpr int CHOL_REG_ROWS = 10;
+// rather than read back from LDS: every row for 64-lane groups (v_readlane into a scalar
+// operand; no LDS round trip, and no registers for LDS copies), the first ten for the
+// other shapes without the DPP-FMA broadcast, which covers the LDS latency of the rest.
+template <int G> struct CholRegRows { static constexpr int value = G == 64 ? 64 : 10; };
 
 template <int G, int S, int R>
 __device__ __forceinline__ constexpr int slot() { return G * S * (S + 1) / 2 + R; }
@@ -421,9 +424,12 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   //                         while row k's updates still read the other one
   //   invd (T) at the end   1/U_kk
   // Row k is written to Upk as it is finished, so the registers of A die row by row.
-  // FP32 16-lane groups broadcast U[k][i] inside the FMAs (v_fmac_f32 with a DPP source);
-  // other shapes publish row k in LDS
+  // FP32 16-lane groups broadcast U[k][i] inside the FMAs (v_fmac_f32 with a DPP source),
+  // 64-lane groups from registers; the FP64 16-lane groups publish row k in LDS for the
+  // rows past the first CRR
   constexpr bool DPP_FMA = std::is_same<T, float>::value && G == 16;
+  constexpr int CRR = CholRegRows<G>::value;
+  constexpr bool PUBLISH = !DPP_FMA && CRR < LMAX - 1;
   cplx<T>* const Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
   cplx<T>* const rowb = Upk + NU + 2;
   T* const invd = reinterpret_cast<T*>(xs_all + (q + 1) * GREG) - LMAX;
@@ -452,7 +458,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       if constexpr (k < G * (S + 1)) {
         A[slot<G, S, k>()].x *= inv;
         A[slot<G, S, k>()].y *= inv;
-        if constexpr (!DPP_FMA) rb[cs[S]] = A[slot<G, S, k>()];
+        if constexpr (PUBLISH) rb[cs[S]] = A[slot<G, S, k>()];
         Upk[k < cs[S] ? k + ucol[S] : NU] = A[slot<G, S, k>()];
       }
     });
@@ -480,7 +486,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       // the first rows take U[k][i] by a register broadcast, which covers the latency of
       // the LDS reads for the rest
       cplx<T> ur[LMAX];
-      static_for<k + 1 + CHOL_REG_ROWS, LMAX>([&](auto Ii_) {
+      static_for<k + 1 + CRR, LMAX>([&](auto Ii_) {
         constexpr int i = decltype(Ii_)::value;
         if (SKIP && i >= Lw) return;
         ur[i] = rb[i];
@@ -488,7 +494,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       static_for<k + 1, LMAX>([&](auto Ii_) {
         constexpr int i = decltype(Ii_)::value;
         if (SKIP && i >= Lw) return;
-        const cplx<T> u = (i <= k + CHOL_REG_ROWS) ? bcastc<G, i % G>(A[slot<G, i / G, k>()]) : ur[i];
+        const cplx<T> u = (i <= k + CRR) ? bcastc<G, i % G>(A[slot<G, i / G, k>()]) : ur[i];
         static_for<0, CPL>([&](auto S_) {
           constexpr int S = decltype(S_)::value;
           if constexpr (i < G * (S + 1)) sub_conj_mul(A[slot<G, S, i>()], u, A[slot<G, S, k>()]);
