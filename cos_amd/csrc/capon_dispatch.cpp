@@ -34,7 +34,8 @@ hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_
     return list ? launch_one<double, 16, 1, 0, true, true, false>(a, Mmax, s) : launch_one<double, 16, 1, 0, true, false, false>(a, Mmax, s);
   if (Lmax <= 32)
     return list ? launch_one<double, 16, 2, 0, true, true, false>(a, Mmax, s) : launch_one<double, 16, 2, 0, true, false, false>(a, Mmax, s);
-  if (Lmax <= 64) return launch_capon_wide_f64(a, Mmax, s);
+  if (Lmax <= 64)   // covariance by the FP64 recursion, the columns in registers
+    return list ? launch_one<double, 64, 1, 0, false, true, false>(a, Mmax, s) : launch_one<double, 64, 1, 0, false, false, false>(a, Mmax, s);
   if (why) *why = "L > 64 is not supported by the FP64 kernel";
   return hipErrorInvalidValue;
 }

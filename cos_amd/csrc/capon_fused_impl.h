@@ -776,6 +776,15 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   }
 }
 
+// The persistent 64-lane kernels call the per-group body out of line, so the
+// grid-stride loop carries no register state across pixels (inlined, the FP64 one
+// spills); the 16-lane ones keep it inline, where the call would cost registers.
+template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD>
+__device__ __noinline__ void capon_group_call(const CaponLaunch& a, const int MP, const int GREG,
+                                              const int gid0, const int npix) {
+  capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
+}
+
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
                           amdgpu_waves_per_eu(BWD ? 1 : OccHint<T, G, CPL, TM>::value)))
@@ -786,8 +795,12 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   constexpr int NGk = Grp<G>::NG;
   const int npix = a.pcount ? min(*a.pcount, a.npix) : a.npix;
   if constexpr (PERSIST) {
-    for (int gid0 = blockIdx.x * NGk; gid0 < npix; gid0 += gridDim.x * NGk)
-      capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
+    for (int gid0 = blockIdx.x * NGk; gid0 < npix; gid0 += gridDim.x * NGk) {
+      if constexpr (G == 64)
+        capon_group_call<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
+      else
+        capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
+    }
   } else {
     // One group of pixels per block (a loop here costs the FP32 kernel its registers).
     // Workgroups are dealt to the 8 XCDs round-robin, and each XCD has its own L2;

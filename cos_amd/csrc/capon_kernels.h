@@ -48,7 +48,5 @@ struct CaponLaunch {
 hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
 hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
 hipError_t launch_capon_bwd(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
-// FP64, 32 < L <= 64: one wave per pixel, matrix in LDS (capon_wide_f64.hip)
-hipError_t launch_capon_wide_f64(const CaponLaunch& a, int Mmax, hipStream_t s);
 
 }  // namespace cosk
