@@ -142,7 +142,7 @@ class CaponPlan:
 
     def run(self, x, d, L, Yavg, *, M=None, off=0, precision="fp32", want=("P", "z", "w"),
             rows=None, steer_sin=None, cond_thresh=1e6, out=None, stream=None, graph=False,
-            fb_avg=False, z_mode="amplitude", p_mode="loaded"):
+            fb_avg=False, z_mode="amplitude", p_mode="loaded", refine=False):
         """Run one frame.
 
         x: complex64 CUDA tensor [Nx][Nz][Mstride].  rows: optional dict of CUDA tensors
@@ -152,8 +152,9 @@ class CaponPlan:
         on the first call and replay it while the descriptor and all tensors are the same
         (pass the same ``out`` dict to keep the pointers).  fb_avg: forward-backward
         averaged covariance; z_mode: "amplitude" (w^H g) or "power" (power Capon);
-        p_mode: "loaded" (1/(a^H R_L^-1 a)) or "unloaded" (|w^H R_hat w|).  Returns a dict
-        of output tensors.
+        p_mode: "loaded" (1/(a^H R_L^-1 a)) or "unloaded" (|w^H R_hat w|).  refine: one step
+        of iterative refinement on the FP32 paths (residual from the snapshots).  Returns a
+        dict of output tensors.
         """
         import torch
 
@@ -171,6 +172,7 @@ class CaponPlan:
         ds.fb_avg = int(bool(fb_avg))
         ds.z_mode = {"amplitude": 0, "power": 1}[z_mode]
         ds.p_mode = {"loaded": 0, "unloaded": 1}[p_mode]
+        ds.refine = int(bool(refine))
         Lmax = int(L)
         keep = []
         if rows is not None:

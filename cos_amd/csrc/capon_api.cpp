@@ -216,6 +216,8 @@ int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
   if (ds.p_mode != COS_P_LOADED && ds.p_mode != COS_P_UNLOADED) return fail(hipErrorInvalidValue, "bad p_mode");
   if (ds.nsteer > 0 && (ds.z_mode != COS_Z_AMPLITUDE || ds.p_mode != COS_P_LOADED))
     return fail(hipErrorInvalidValue, "steering takes the default z and P definitions");
+  if (ds.refine && (ds.precision == COS_PREC_FP64 || ds.nsteer > 0 || ds.fb_avg))
+    return fail(hipErrorInvalidValue, "refinement applies to the FP32 paths without steering or averaging");
   return 0;
 }
 
@@ -244,6 +246,7 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
   a.wstride = wstride;
   a.R = (float2*)R;
   a.Rstride = ds.L * ds.L;
+  a.refine = ds.refine ? 1 : 0;
   const char* why = nullptr;
   hipError_t e;
   if (ds.precision == COS_PREC_FP64) {

@@ -92,7 +92,7 @@ template <typename T, int G, int CPL, int TM> struct OccHint {
                                               : (G == 16 && CPL == 1 ? 2 : 1);
 };
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0>
+template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0, bool REF = false>
 __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, const int GREG,
                                             const int gid0, const int npix) {
   using Gp = Grp<G>;
@@ -513,10 +513,61 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     notpd = !(maxinv < T(0.5) * rsqrt_t<T>(tiny_value<T>()));
   }
 
-  const bool want_y = a.w || a.z || BWD || (OPTS && a.pmode == 1 && a.P);
+  const bool want_y = a.w || a.z || BWD || REF || (OPTS && a.pmode == 1 && a.P);
   const T norm = (T)K * (T)TW;
   const int nrhs = steered ? a.nsteer : 1;
   const float2* xcg = a.x + ((size_t)ix * a.Nz + (j + Y)) * a.Mstride + off;   // centre row (steering)
+
+  // Forward solve U^H v = b with the rows of U read back from Upk; v_k goes to the
+  // diagonal slot of column k.  Lanes whose column is <= k read the dump slot and change
+  // only right-hand sides already consumed.
+  auto fwd_upk = [&](cplx<T>(&bs)[CPL]) {
+    static_for<0, LMAX>([&](auto Kk_) {
+      constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
+      if (SKIP && k >= Lw) return;
+      cplx<T> v1 = bcastc<G, lk>(bs[sk]);
+      const T inv = invd[k];
+      v1.x *= inv;
+      v1.y *= inv;
+      static_for<0, CPL>([&](auto S_) {
+        constexpr int S = decltype(S_)::value;
+        if constexpr (k < G * (S + 1)) {
+          const cplx<T> u = Upk[k < cs[S] ? k + ucol[S] : NU];
+          sub_conj_mul(bs[S], u, v1);
+        }
+      });
+      Upk[k + k * (k + 1) / 2] = v1;
+    });
+  };
+  // Back solve U y = v with v in cr, column-oriented: y_j = c_j / U_jj, then
+  // c_i -= U[i][j] y_j for the rows i < j this lane owns (U[i][j] read from Upk; lanes
+  // whose row is >= j read the zeroed diagonal slot of column j and leave c_i unchanged).
+  auto back_upk = [&](cplx<T>(&cr)[CPL], cplx<T>(&yo)[CPL]) {
+    static_for<0, CPL>([&](auto S_) {
+      constexpr int S = decltype(S_)::value;
+      Upk[cs[S] + ucol[S]] = cplx<T>{T(0), T(0)};
+    });
+    static_for<LMAX - 1, 0, -1>([&](auto Cc_) {
+      constexpr int jj = decltype(Cc_)::value, sj = jj / G, lj = jj % G;
+      if (SKIP && jj >= Lw) return;
+      const T inv = invd[jj];
+      cplx<T> yb = bcastc<G, lj>(cr[sj]);
+      yb.x *= inv;
+      yb.y *= inv;
+      static_for<0, CPL>([&](auto S_) {
+        constexpr int S = decltype(S_)::value;
+        if constexpr (G * S < jj) {                      // some row i = g + G*S < jj
+          const cplx<T> u = Upk[min(cs[S], jj) + jj * (jj + 1) / 2];
+          sub_mul(cr[S], u, yb);
+        }
+      });
+    });
+    static_for<0, CPL>([&](auto S_) {
+      constexpr int S = decltype(S_)::value;
+      const T inv = cs[S] < Lw ? invd[cs[S]] : T(0);
+      yo[S] = {cr[S].x * inv, cr[S].y * inv};
+    });
+  };
 
   for (int ia = 0; ia < nrhs; ++ia) {
     const T sn = steered ? (T)a.steer_sin[ia] : T(0);
@@ -537,22 +588,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         bs[S] = cc < L ? cplx<T>{(T)cv, (T)sv} : cplx<T>{T(0), T(0)};
         gcur[S] = steered_mean<T>(xcg, cc, K, L, sn, invK);
       });
-      static_for<0, LMAX>([&](auto Kk_) {
-        constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
-        if (SKIP && k >= Lw) return;
-        cplx<T> v1 = bcastc<G, lk>(bs[sk]);
-        const T inv = invd[k];
-        v1.x *= inv;
-        v1.y *= inv;
-        static_for<0, CPL>([&](auto S_) {
-          constexpr int S = decltype(S_)::value;
-          if constexpr (k < G * (S + 1)) {
-            const cplx<T> u = Upk[k < cs[S] ? k + ucol[S] : NU];
-            sub_conj_mul(bs[S], u, v1);
-          }
-        });
-        Upk[k + k * (k + 1) / 2] = v1;
-      });
+      fwd_upk(bs);
     }
 
     // v (the forward solution, one entry per column) sits in the diagonal slots;
@@ -568,38 +604,107 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 
     const size_t po = (size_t)p * nrhs + ia;
     const float nanv = __builtin_nanf("");
-    if (active && g == 0 && a.P && (!OPTS || a.pmode == 0)) a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
+    if (!REF && active && g == 0 && a.P && (!OPTS || a.pmode == 0))
+      a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
 
     if (want_y) {
-      // back solve U y = v, column-oriented: y_j = c_j / U_jj, then c_i -= U[i][j] y_j
-      // for the rows i < j this lane owns (U[i][j] read from Upk; lanes whose row is
-      // >= j read the zeroed diagonal slot of column j and leave c_i unchanged).
-      static_for<0, CPL>([&](auto S_) {
-        constexpr int S = decltype(S_)::value;
-        Upk[cs[S] + ucol[S]] = cplx<T>{T(0), T(0)};
-      });
-      static_for<LMAX - 1, 0, -1>([&](auto Cc_) {
-        constexpr int jj = decltype(Cc_)::value, sj = jj / G, lj = jj % G;
-        if (SKIP && jj >= Lw) return;
-        const T inv = invd[jj];
-        cplx<T> yb = bcastc<G, lj>(cr[sj]);
-        yb.x *= inv;
-        yb.y *= inv;
+      cplx<T> y[CPL];
+      back_upk(cr, y);
+      if constexpr (REF) {
+        // One step of iterative refinement against the covariance of the data itself:
+        // r = a - S y - delta y, with S y = sum_t sum_k u_{t,k} beta_{t,k},
+        // beta_{t,k} = u_{t,k}^H y, over the subarrays u_{t,k}[l] = x_t[k+l] of the window
+        // (so the residual sees neither the recursion's nor the factorisation's rounding),
+        // then S_L dy = r by the two triangular solves with U from LDS, y <- y + dy.
+        // The window rows are read again from global memory (cache-resident), so the
+        // refinement needs no LDS beyond the factorisation's.  Lane g takes subarray
+        // k = kb + g of each block of G and loads its window xv[l] = x_t[kb + g + l],
+        // l < LMAX, once: the same registers give beta_k (a dot product with y, y_l
+        // broadcast inside the FMAs) and, since x_t[kb + l2 + g + G s] = xv[l2 + G s], the
+        // terms x_t[k + c] beta_k of the lane's columns c = g + G s (beta_k broadcast from
+        // lane l2).  Indices are clamped to the row's last active element; a clamped read
+        // only meets a zero (y_l for l >= L, beta_k for k >= K) or a padding column,
+        // whose residual is dropped.
+        const float2* srcw = a.x + ((size_t)ix * a.Nz + j) * a.Mstride + off;
+        cplx<T> sy[CPL];
         static_for<0, CPL>([&](auto S_) {
           constexpr int S = decltype(S_)::value;
-          if constexpr (G * S < jj) {                      // some row i = g + G*S < jj
-            const cplx<T> u = Upk[min(cs[S], jj) + jj * (jj + 1) / 2];
-            sub_mul(cr[S], u, yb);
-          }
+          sy[S] = {T(0), T(0)};
         });
-      });
-      cplx<T> y[CPL];
-      static_for<0, CPL>([&](auto S_) {
-        constexpr int S = decltype(S_)::value;
-        const T inv = cs[S] < Lw ? invd[cs[S]] : T(0);
-        y[S] = {cr[S].x * inv, cr[S].y * inv};
-      });
-      // w = y / (a^H y), a^H y = |v|^2;  z = w^H g
+        for (int t = 0; t < TW; ++t) {
+          const float2* xt = srcw + (size_t)t * a.Mstride;
+          for (int kb = 0; kb < K; kb += G) {
+            if (K - kb < G / 4) {
+              // a short last block (K = 33 at L = 32 leaves one subarray): each beta_k
+              // as a group sum over the lanes' columns, x_t[k + c] loaded once for both
+              for (int k = kb; k < K; ++k) {
+                cplx<T> xk[CPL];
+                T br = T(0), bi = T(0);
+                static_for<0, CPL>([&](auto S_) {
+                  constexpr int S = decltype(S_)::value;
+                  xk[S] = ldx<T>(xt + min(k + cs[S], M - 1));
+                  br = fma(xk[S].x, y[S].x, fma(xk[S].y, y[S].y, br));
+                  bi = fma(xk[S].x, y[S].y, fma(-xk[S].y, y[S].x, bi));
+                });
+                const cplx<T> bk = {Gp::sum(br), Gp::sum(bi)};
+                static_for<0, CPL>([&](auto S_) {
+                  constexpr int S = decltype(S_)::value;
+                  add_mul(sy[S], xk[S], bk);
+                });
+              }
+              break;
+            }
+            // 64-lane groups: beta needs l < L, the column terms l2 < K - kb (uniform)
+            const int nk = SKIP ? __builtin_amdgcn_readfirstlane(K - kb) : G;
+            cplx<T> xv[LMAX];
+            static_for<0, LMAX>([&](auto l_) {
+              constexpr int l = decltype(l_)::value;
+              if (SKIP && l >= Lw && l >= nk) return;
+              xv[l] = ldx<T>(xt + min(kb + g + l, M - 1));
+            });
+            cplx<T> nb = {T(0), T(0)};   // -(y^H u_{t,k}) = -conj(beta_{t,k})
+            static_for<0, LMAX>([&](auto l_) {
+              constexpr int l = decltype(l_)::value;
+              if (SKIP && l >= Lw) return;
+              bcast_sub_conj_mul<G, l % G>(nb, y[l / G], xv[l]);
+            });
+            if (kb + g >= K) nb = {T(0), T(0)};
+            // (S y)_c += x_t[k + c] beta_{t,k} for the G subarrays of the block
+            static_for<0, G>([&](auto l2_) {
+              constexpr int l2 = decltype(l2_)::value;
+              if (SKIP && l2 >= nk) return;   // k = kb + l2 >= K: beta_k = 0
+              if constexpr (CPL == 2)
+                bcast_sub_conj_mul2<G, l2>(sy[0], sy[1], nb, xv[l2], xv[l2 + G]);
+              else
+                bcast_sub_conj_mul<G, l2>(sy[0], nb, xv[l2]);
+            });
+          }
+        }
+        cplx<T> r[CPL];
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          r[S] = cs[S] < L ? cplx<T>{T(1) - sy[S].x - delta * y[S].x, -sy[S].y - delta * y[S].y}
+                           : cplx<T>{T(0), T(0)};
+        });
+        fwd_upk(r);
+        cplx<T> cr2[CPL], dy[CPL];
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          cr2[S] = cs[S] < Lw ? Upk[cs[S] + ucol[S]] : cplx<T>{T(0), T(0)};
+        });
+        back_upk(cr2, dy);
+        T ay = T(0);
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          y[S].x += dy[S].x;
+          y[S].y += dy[S].y;
+          if (cs[S] < L) ay += y[S].x;
+        });
+        accP = Gp::sum(ay);   // a^H y (real part) of the refined solution
+        if (active && g == 0 && a.P && (!OPTS || a.pmode == 0))
+          a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
+      }
+      // w = y / (a^H y), a^H y = |v|^2 (refined: its real part);  z = w^H g
       const T sc = T(1) / accP;
       cplx<T> zp = {T(0), T(0)};
       static_for<0, CPL>([&](auto S_) {
@@ -779,13 +884,13 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 // The persistent 64-lane kernels call the per-group body out of line, so the
 // grid-stride loop carries no register state across pixels (inlined, the FP64 one
 // spills); the 16-lane ones keep it inline, where the call would cost registers.
-template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD>
+template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD, bool REF>
 __device__ __noinline__ void capon_group_call(const CaponLaunch& a, const int MP, const int GREG,
                                               const int gid0, const int npix) {
-  capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
+  capon_group<T, G, CPL, TM, DIRECT, BWD, REF>(a, MP, GREG, gid0, npix);
 }
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
                           amdgpu_waves_per_eu(BWD ? 1 : OccHint<T, G, CPL, TM>::value)))
 void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
@@ -797,9 +902,9 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   if constexpr (PERSIST) {
     for (int gid0 = blockIdx.x * NGk; gid0 < npix; gid0 += gridDim.x * NGk) {
       if constexpr (G == 64)
-        capon_group_call<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
+        capon_group_call<T, G, CPL, TM, DIRECT, BWD, REF>(a, MP, GREG, gid0, npix);
       else
-        capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
+        capon_group<T, G, CPL, TM, DIRECT, BWD, REF>(a, MP, GREG, gid0, npix);
     }
   } else {
     // One group of pixels per block (a loop here costs the FP32 kernel its registers).
@@ -810,12 +915,12 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
     const unsigned B = gridDim.x, B8 = B / NXCD * NXCD, b = blockIdx.x;
     const unsigned lb = b < B8 ? (b % NXCD) * (B8 / NXCD) + b / NXCD : b;
     const int gid0 = (int)lb * NGk;
-    if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT, BWD>(a, MP, GREG, gid0, npix);
+    if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT, BWD, REF>(a, MP, GREG, gid0, npix);
   }
 }
 
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   constexpr int NG = 64 / G, LMAX = G * CPL;
   const int MP = ((Mmax + LMAX + 4 + 3) / 4) * 4;   // zero padding covers every lag read
@@ -837,7 +942,7 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   int blocks = (a.npix + NG - 1) / NG;
   if (PERSIST && blocks > 2048) blocks = 2048;   // list length known only on the device
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST, BWD>), dim3(blocks), dim3(64), lds, s, a, MP, GREG);
+  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST, BWD, REF>), dim3(blocks), dim3(64), lds, s, a, MP, GREG);
   return hipGetLastError();
 }
 

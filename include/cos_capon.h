@@ -100,6 +100,9 @@ typedef struct {
                             (getCaponAllC.pyx:357-358) */
     int z_mode;          /* COS_Z_*: amplitude output */
     int p_mode;          /* COS_P_*: power output */
+    int refine;          /* nonzero: one step of iterative refinement on the FP32 paths, the
+                            residual a - R_L y taken from the snapshots themselves (no
+                            steering, no forward-backward averaging) */
 } cos_capon_desc;
 
 enum {

@@ -70,6 +70,11 @@ def test_plan_run_rejects_bad_descriptor():
     ds = CaponDesc(Nx=1, Nz=8, Mstride=1 << 22, M=8, L=4, off=0, Yavg=1, d=0.1)
     code, text = _err(L.cos_capon_beamspace_run(h, C.byref(ds), 3, 16, None, None, None, None, 0, None))
     assert code != 0 and "Mstride < 4194304" in text
+    # refinement is defined for the FP32 paths with the unsteered, unaveraged covariance
+    for kw in (dict(precision=_lib.COS_PREC_FP64), dict(fb_avg=1)):
+        ds = CaponDesc(Nx=1, Nz=8, Mstride=8, M=8, L=4, off=0, Yavg=1, d=0.1, refine=1, **kw)
+        code, text = _err(L.cos_capon_run(h, C.byref(ds), 16, None, None, None, 0, None, None, None))
+        assert code != 0 and "refinement applies to the FP32 paths" in text
     assert L.cos_capon_plan_destroy(h) == 0
 
 

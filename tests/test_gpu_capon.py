@@ -112,6 +112,97 @@ def test_cfg2_slice_fp64_kernel(cfg2_slice):
     np.testing.assert_allclose(o["w"], ref["w"].reshape(-1, 32), rtol=0, atol=2e-7 * np.abs(ref["w"]).max())
 
 
+# ------------------------------------------------------------------ iterative refinement
+
+def test_cfg2_slice_refined_power(cfg2_slice):
+    """One refinement step (residual from the snapshots) on the 64-element, L = 32 slice:
+    power within 1e-4 of FP64 and no worse than the unrefined FP32 result."""
+    say("cfg2 slice fp32 refine", 8, 2048, 64, 32)
+    x, ref = cfg2_slice
+    o0, _ = _plan_run(x, 0.01, 32, 2)
+    o1, _ = _plan_run(x, 0.01, 32, 2, refine=True)
+    e0, e1 = rel(o0["P"], ref["P"].ravel()), rel(o1["P"], ref["P"].ravel())
+    w64 = ref["w"].reshape(-1, 32)
+    we = lambda w: np.linalg.norm(w - w64, axis=1) / np.linalg.norm(w64, axis=1)
+    print(f"P err max {e0.max():.3e} -> {e1.max():.3e}, mean {e0.mean():.3e} -> {e1.mean():.3e}; "
+          f"w err max {we(o0['w']).max():.3e} -> {we(o1['w']).max():.3e}", flush=True)
+    assert e1.max() <= 1e-4
+    assert e1.max() <= max(e0.max(), 2e-6)
+    assert we(o1["w"]).max() <= 1e-3
+    ze = np.abs(o1["z"] - ref["z"].ravel()) / np.abs(ref["z"].ravel()).max()
+    assert ze.max() <= 1e-4
+
+
+def test_refinement_on_ill_conditioned_covariance():
+    """d = 1e-4 (condition bound 1 + L/d = 3.2e5, below the fallback threshold): the FP32
+    solve loses digits; one refinement step must bring power and weights closer to FP64."""
+    from cos_amd import synth
+    say("refine ill-conditioned", 4, 64, 64, 32)
+    x = synth.config2(Nx=4, Nz=64, M=64, seed=12)
+    ref = c_capon(x, 1e-4, 32, 2)
+    o0, _ = _plan_run(x, 1e-4, 32, 2)
+    o1, _ = _plan_run(x, 1e-4, 32, 2, refine=True)
+    e0, e1 = rel(o0["P"], ref["P"].ravel()), rel(o1["P"], ref["P"].ravel())
+    w64 = ref["w"].reshape(-1, 32)
+    we = lambda w: np.linalg.norm(w - w64, axis=1) / np.linalg.norm(w64, axis=1)
+    print(f"P err max {e0.max():.3e} -> {e1.max():.3e}; w err max {we(o0['w']).max():.3e} -> "
+          f"{we(o1['w']).max():.3e}", flush=True)
+    assert e1.max() <= 1e-4
+    assert e1.mean() <= e0.mean()
+    assert np.median(we(o1["w"])) <= np.median(we(o0["w"]))
+
+
+@pytest.mark.parametrize("M,L,Y", [(24, 12, 1), (48, 24, 2), (80, 40, 1), (128, 64, 2), (16, 1, 2)])
+def test_refined_shapes_match_oracle(M, L, Y):
+    """Every lane shape of the refined kernel (16 lanes x 1 or 2 columns, 64 lanes) against
+    the FP64 oracle: P within 1e-4, w within 1e-3 of its largest entry, z within 1e-4."""
+    say("refine shape", M, L, Y)
+    x = uniform_cube(3 * M + L, 3, 14, M)
+    o, _ = _plan_run(x, 0.05, L, Y, refine=True)
+    ref = O.capon(x.astype(np.complex128), 0.05, L, Y)
+    eP = rel(o["P"], ref["P"].ravel())
+    ez = np.abs(o["z"] - ref["z"].ravel()) / np.abs(ref["z"]).max()
+    print(f"P err {eP.max():.2e} z err {ez.max():.2e}", flush=True)
+    assert eP.max() <= 1e-4
+    assert ez.max() <= 1e-4
+    np.testing.assert_allclose(o["w"], ref["w"].reshape(-1, L), atol=1e-3 * np.abs(ref["w"]).max())
+
+
+def test_refined_unloaded_power_and_power_amplitude():
+    say("refine options", 48, 24, 2)
+    x = uniform_cube(77, 3, 12, 48)
+    o, _ = _plan_run(x, 0.05, 24, 2, refine=True, z_mode="power", p_mode="unloaded")
+    ref = O.capon_options(x.astype(np.complex128), 0.05, 24, 2, fb=False, z_mode=1, p_mode=1)
+    assert rel(o["P"], ref["P"]).max() <= 1e-4
+    assert (np.abs(o["z"] - ref["z"]) / np.abs(ref["z"]).max()).max() <= 1e-4
+
+
+def test_refined_rows_tables_with_fallback():
+    """Per-row apertures (L 8..32) with 7 rows at d = 1e-5: the refined FP32 runs, then the
+    FP64 pass over the flagged pixels."""
+    import torch
+    from cos_amd import synth
+    say("refine rows+fallback", 4, 96, 64)
+    Nx, Nz, Mmax, Y = 4, 96, 64, 2
+    Mz = (2 * np.round((16 + (Mmax - 16) * np.arange(Nz) / (Nz - 1)) / 2)).astype(np.int32)
+    Lz = (Mz // 2).astype(np.int32)
+    off = ((Mmax - Mz) // 2).astype(np.int32)
+    dz = np.full(Nz, 0.01, np.float32)
+    dz[10:14] = 1e-5
+    dz[60:63] = 1e-5
+    x = synth.speckle_cube(Nx, Nz, Mmax, seed=31)
+    rows = {k: torch.from_numpy(v).cuda() for k, v in dict(L=Lz, M=Mz, off=off, d=dz).items()}
+    o, plan = _plan_run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", refine=True,
+                        want=("P", "z", "w", "flags"))
+    ref = O.capon_rows(x.astype(np.complex128), dz.astype(np.float64), Lz, Mz, off, Y)
+    e = rel(o["P"], ref["P"])
+    print(f"refined rows P rel err max {e.max():.3e}", flush=True)
+    assert e.max() <= 1e-4
+    flagged = np.nonzero(o["flags"])[0]
+    assert plan.fallback_count() == len(flagged) > 0
+    assert rel(o["P"][flagged], ref["P"][flagged]).max() <= 1.2e-7
+
+
 # ------------------------------------------------------------------ per-row apertures and fallback
 
 def test_rows_tables_and_fp64_fallback():
