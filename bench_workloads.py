@@ -16,6 +16,8 @@ samples.
 """
 from __future__ import annotations
 
+import time
+
 import numpy as np
 
 PEAK_FP32_TFLOPS = 157.3
@@ -54,7 +56,15 @@ def aperture(args):
     npx = Nx * (Nz - 2 * Y)
     out = {"P": torch.empty(npx, device="cuda"), "z": torch.empty(npx, dtype=torch.complex64, device="cuda"),
            "w": torch.empty((npx, 64), dtype=torch.complex64, device="cuda")}
-    step = lambda: plan.run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", out=out)
+    step = lambda refine=False: plan.run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", out=out,
+                                         refine=refine)
+    for _ in range(args.warmup):
+        step()
+    ms_default = _timed(torch, step, args.steps)
+    # the lane-shape autotuner: every shape that holds each L, timed on up to 128 rows of it
+    t0 = time.perf_counter()
+    tuned = plan.autotune(x, 0.0, 0, Y, rows=rows)
+    tune_s = time.perf_counter() - t0
     for _ in range(args.warmup):
         step()
     ms = _timed(torch, step, args.steps)
@@ -63,6 +73,9 @@ def aperture(args):
     o64 = {"P": torch.empty(npx, device="cuda")}
     ms64 = _timed(torch, lambda: plan.run(x, 0.0, 0, Y, rows=rows, precision="fp64", out=o64, want=("P",)), 1)
     e = (P32.double() - o64["P"].double()).abs() / o64["P"].double()
+    # one step of iterative refinement on the FP32 runs (FP64 fallback as before)
+    ms_ref = _timed(torch, lambda: step(True), max(3, args.steps // 4))
+    er = (out["P"].double() - o64["P"].double()).abs() / o64["P"].double()
     c = np.arange(Nz - 2 * Y) + Y
     fpx = float(np.mean([_flops(int(Lz[cc]), int(Mz[cc] - Lz[cc] + 1)) for cc in c]))
     # oracle sample: 4 beams x 16 rows spread over the depth (FP64 numpy)
@@ -85,7 +98,14 @@ def aperture(args):
     line["speedup_vs_all_fp64"] = round(ms64 / ms, 2)
     line["accuracy"] = {"vs_all_fp64_P_rel_err_max": float(e.max()),
                         "vs_all_fp64_P_rel_err_p999": float(torch.quantile(e[::97].float(), 0.999)),
+                        "vs_all_fp64_P_rel_err_mean": float(e.mean()),
                         "vs_oracle_P_rel_err_max_64px": float(eo.max())}
+    default = {l: (1 if l <= 16 else 2 if l <= 32 else 3) for l in tuned["shapes"]}
+    line["autotune"] = {"default_shapes_ms_per_frame": round(ms_default, 3), "tuning_s": round(tune_s, 2),
+                        "lengths": len(tuned["shapes"]),
+                        "changed": {l: s for l, s in tuned["shapes"].items() if s != default[l]}}
+    line["refined"] = {"ms_per_frame": round(ms_ref, 3), "vs_all_fp64_P_rel_err_max": float(er.max()),
+                       "vs_all_fp64_P_rel_err_mean": float(er.mean())}
     return line
 
 

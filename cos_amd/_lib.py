@@ -15,7 +15,8 @@ EXPORTS = (
     "cos_capon_get_capon", "cos_capon_get_capon_p", "cos_buildr_get_r", "cos_solver_solve",
     "cos_capon_last_error", "cos_capon_plan_create", "cos_capon_plan_destroy", "cos_capon_run",
     "cos_capon_last_fallback_count", "cos_capon_beamspace_run", "cos_capon_backward",
-    "cos_capon_subspace_run", "cos_tdib_run", "cos_lca_run",
+    "cos_capon_subspace_run", "cos_tdib_run", "cos_lca_run", "cos_capon_autotune",
+    "cos_capon_tuning_set",
 )
 
 COS_PREC_FP32 = 0
@@ -76,6 +77,9 @@ def lib() -> C.CDLL:
     L.cos_capon_beamspace_run.argtypes = [vp, C.POINTER(CaponDesc), ip, vp, vp, vp, vp, vp, ip, vp]
     L.cos_capon_backward.argtypes = [vp, C.POINTER(CaponDesc), vp, vp, vp, vp, vp]
     L.cos_capon_subspace_run.argtypes = [vp, C.POINTER(CaponDesc), vp, ip, vp, vp, vp, vp, vp]
+    L.cos_capon_autotune.argtypes = [vp, C.POINTER(CaponDesc), vp, vp, vp, vp, ip, vp,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_float)]
+    L.cos_capon_tuning_set.argtypes = [C.POINTER(CaponDesc), C.POINTER(C.c_int)]
     db = C.c_double
     L.cos_tdib_run.argtypes = [vp, ip, ip, db, db, vp, vp, db, db, vp, vp, ip, ip, fl, vp, vp, vp]
     L.cos_lca_run.argtypes = [vp, ip, ip, ip, vp, ip, ip, vp, vp, vp]

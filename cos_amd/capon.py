@@ -156,6 +156,18 @@ class CaponPlan:
         of iterative refinement on the FP32 paths (residual from the snapshots).  Returns a
         dict of output tensors.
         """
+        ds, o, ptr, wstride, stream = self._prepare(
+            x, d, L, Yavg, M=M, off=off, precision=precision, want=want, rows=rows, steer_sin=steer_sin,
+            cond_thresh=cond_thresh, out=out, stream=stream, graph=graph, fb_avg=fb_avg, z_mode=z_mode,
+            p_mode=p_mode, refine=refine)
+        check(lib().cos_capon_run(self._h, C.byref(ds), x.data_ptr(), ptr("P"), ptr("z"), ptr("w"),
+                                  wstride, ptr("R"), ptr("flags"), stream))
+        return o
+
+    @staticmethod
+    def _desc(x, d, L, Yavg, *, M=None, off=0, precision="fp32", rows=None, cond_thresh=1e6, graph=False,
+              fb_avg=False, z_mode="amplitude", p_mode="loaded", refine=False):
+        """The cos_capon_desc of a frame (see run())."""
         import torch
 
         if not (x.is_cuda and x.dtype == torch.complex64 and x.is_contiguous()):
@@ -174,7 +186,6 @@ class CaponPlan:
         ds.p_mode = {"loaded": 0, "unloaded": 1}[p_mode]
         ds.refine = int(bool(refine))
         Lmax = int(L)
-        keep = []
         if rows is not None:
             for k in ("L", "M", "off", "d"):
                 t = rows[k]
@@ -184,6 +195,17 @@ class CaponPlan:
             ds.row_off, ds.row_d = rows["off"].data_ptr(), rows["d"].data_ptr()
             Lmax = int(rows["L"].max().item())
             ds.Lmax, ds.Mmax = Lmax, int(rows["M"].max().item())
+        return ds, Lmax
+
+    def _prepare(self, x, d, L, Yavg, *, M, off, precision, want, rows, steer_sin, cond_thresh, out, stream,
+                 graph, fb_avg, z_mode, p_mode, refine):
+        """The frame descriptor and output tensors shared by run() and autotune()."""
+        import torch
+
+        ds, Lmax = CaponPlan._desc(x, d, L, Yavg, M=M, off=off, precision=precision, rows=rows,
+                                   cond_thresh=cond_thresh, graph=graph, fb_avg=fb_avg, z_mode=z_mode,
+                                   p_mode=p_mode, refine=refine)
+        Nx, Nz, Mstride = x.shape
         S = 1
         if steer_sin is not None:
             ds.steer_sin, ds.nsteer = steer_sin.data_ptr(), int(steer_sin.numel())
@@ -205,9 +227,41 @@ class CaponPlan:
         wstride = o["w"].shape[-1] if "w" in o else 0
         if stream is None:
             stream = torch.cuda.current_stream(dev).cuda_stream
-        check(lib().cos_capon_run(self._h, C.byref(ds), x.data_ptr(), ptr("P"), ptr("z"), ptr("w"),
-                                  wstride, ptr("R"), ptr("flags"), stream))
-        return o
+        return ds, o, ptr, wstride, stream
+
+    def autotune(self, x, d, L, Yavg, *, M=None, off=0, rows=None, precision="fp32+fallback", fb_avg=False,
+                 z_mode="amplitude", p_mode="loaded", refine=False, stream=None):
+        """Time every lane shape of the fused FP32 kernel for each subarray length of this
+        frame configuration and install the fastest for the device (later run() calls with
+        the same Mmax, Yavg and options use it).  Arguments as run().  Returns
+        {"shapes": {L: shape}, "ms": {L: {shape: ms per launch}}}; shape 1 = 16 lanes x 1
+        column, 2 = 16 lanes x 2 columns, 3 = 64 lanes."""
+        ds, o, ptr, wstride, stream = self._prepare(
+            x, d, L, Yavg, M=M, off=off, precision=precision, want=("P", "z", "w"), rows=rows, steer_sin=None,
+            cond_thresh=1e6, out=None, stream=stream, graph=False, fb_avg=fb_avg, z_mode=z_mode,
+            p_mode=p_mode, refine=refine)
+        shapes = (C.c_int * 65)()
+        ms = (C.c_float * (65 * 3))()
+        check(lib().cos_capon_autotune(self._h, C.byref(ds), x.data_ptr(), ptr("P"), ptr("z"), ptr("w"),
+                                       wstride, stream, shapes, ms))
+        res = {"shapes": {}, "ms": {}}
+        for l in range(1, 65):
+            if shapes[l]:
+                res["shapes"][l] = int(shapes[l])
+                res["ms"][l] = {s + 1: float(ms[3 * l + s]) for s in range(3) if ms[3 * l + s] >= 0}
+        return res
+
+    @staticmethod
+    def set_tuning(x, d, L, Yavg, shapes, *, M=None, off=0, rows=None, precision="fp32+fallback", fb_avg=False,
+                   z_mode="amplitude", p_mode="loaded", refine=False):
+        """Install a lane-shape table {L: shape} (e.g. autotune()["shapes"], saved and
+        reloaded) for the device and the frame configuration described as for run()."""
+        ds, _ = CaponPlan._desc(x, d, L, Yavg, M=M, off=off, precision=precision, rows=rows, fb_avg=fb_avg,
+                                z_mode=z_mode, p_mode=p_mode, refine=refine)
+        tab = (C.c_int * 65)()
+        for l, sh in shapes.items():
+            tab[int(l)] = int(sh)
+        check(lib().cos_capon_tuning_set(C.byref(ds), tab))
 
     def run_beamspace(self, x, d, L, Yavg, Nb, *, M=None, off=0, want=("P", "z"), out=None, stream=None):
         """Beamspace Capon with the sliding DFT over Nb Butler beams (loading in beamspace).

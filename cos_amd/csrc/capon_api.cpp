@@ -7,10 +7,15 @@
 // take device pointers and a stream and only enqueue work.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/cos_capon.h"
@@ -56,12 +61,42 @@ struct DevBuf {
 
 // Output rows jlo .. jlo+jn-1 whose centre-row L falls in one lane-shape bucket.
 struct RowRun {
-  int jlo, jn, Lmax;
+  int jlo, jn, Lmax, shape;
 };
 
-// Lane-shape buckets of the fused kernel: 16-lane groups with one or two columns per
-// lane, or a whole wave per pixel.
-inline int bucket_cap(int L) { return L <= 16 ? 16 : L <= 32 ? 32 : 64; }
+// Autotuned lane shapes: for each device and each frame configuration that changes the
+// kernels' cost (largest aperture, window, options), the fastest shape per subarray
+// length L = 1..64 as measured by cos_capon_autotune (0 = not tuned: the smallest shape
+// that holds L).  Process-wide, so every plan on the device uses it; g_tune_gen counts
+// changes so captured frame graphs are re-captured after one.
+using TuneKey = std::tuple<int, int, int, int, int, int, int>;   // device, Mmax, Yavg, fb, z, p, refine
+using TuneTable = std::array<unsigned char, 65>;
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneTable> g_tune;
+std::atomic<int> g_tune_gen{0};
+
+TuneKey tune_key(const cos_capon_desc& ds) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  return TuneKey{dev, tables ? ds.Mmax : ds.M, ds.Yavg, ds.fb_avg ? 1 : 0, ds.z_mode, ds.p_mode, ds.refine ? 1 : 0};
+}
+
+// Shapes for this frame: the tuned table of its key, or all zeros.
+TuneTable tuned_shapes(const cos_capon_desc& ds) {
+  TuneTable t{};
+  if (ds.precision == COS_PREC_FP64) return t;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(tune_key(ds));
+  if (it != g_tune.end()) t = it->second;
+  return t;
+}
+
+inline int shape_for(const TuneTable& t, int L) {
+  if (L < 1 || L > 64) return cosk::default_shape(L < 1 ? 1 : 64);
+  const int s = t[L];
+  return s >= 1 && s <= cosk::kShapes && cosk::shape_cap(s) >= L ? s : cosk::default_shape(L);
+}
 
 // The frame's launches captured once and replayed: valid while the descriptor, every
 // pointer and wstride are unchanged.
@@ -70,6 +105,7 @@ struct FrameGraph {
   cos_capon_desc ds;
   const void* ptr[7];
   int wstride = 0;
+  int tune_gen = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   void reset() {
@@ -89,7 +125,8 @@ struct cos_capon_plan {
   hipStream_t last_stream = nullptr;
   hipStream_t capture_stream = nullptr;  // graphs are captured here, launched on the caller's
   FrameGraph fg;
-  std::vector<RowRun> runs;   // L-bucket runs of output rows for table-driven frames
+  std::vector<RowRun> runs;   // lane-shape runs of output rows for table-driven frames
+  TuneTable shapes{};         // the tuned shapes the current frame uses
   bool fallback_ran = false;
   ~cos_capon_plan() {
     fg.reset();
@@ -276,11 +313,11 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
       b.jlo = r.jlo;
       b.jn = r.jn;
       b.npix = ds.Nx * r.jn;
-      e = cosk::launch_capon_f32(b, r.Lmax, Mmax, s, &why);
+      e = cosk::launch_capon_f32_shape(b, r.shape, Mmax, s, &why);
       if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
     }
   } else {
-    e = cosk::launch_capon_f32(a, Lmax, Mmax, s, &why);
+    e = cosk::launch_capon_f32_shape(a, shape_for(plan->shapes, Lmax), Mmax, s, &why);
     if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
   }
   if (!fb) return 0;
@@ -321,24 +358,28 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   plan->fallback_ran = fb;
   const void* ptr[7] = {x, P, z, w, R, flags, nullptr};
   FrameGraph& g = plan->fg;
+  const int gen = g_tune_gen.load();
   const bool replay = ds.graph && g.valid && same_desc(g.ds, ds) && std::memcmp(g.ptr, ptr, sizeof(ptr)) == 0 &&
-                      g.wstride == wstride;
+                      g.wstride == wstride && g.tune_gen == gen;
   plan->runs.clear();
+  plan->shapes = tuned_shapes(ds);
   const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
   if (tables && !replay) {
-    // Read the L table (Nz ints) to split the rows into L-bucket runs; more than 16 runs
-    // fall back to a single launch sized for Lmax.
-    // (ordered after the caller's work on s, which may still be writing the table)
+    // Read the L table (Nz ints) to split the rows into runs of one lane shape (the tuned
+    // one, or the smallest that holds L); more than 16 runs fall back to a single launch
+    // sized for Lmax.  (ordered after the caller's work on s, which may still be writing
+    // the table)
     std::vector<int> hL(ds.Nz);
     if ((e = hipMemcpyAsync(hL.data(), ds.row_L, sizeof(int) * ds.Nz, hipMemcpyDeviceToHost, s)) != hipSuccess ||
         (e = hipStreamSynchronize(s)) != hipSuccess)
       return fail_hip(e, "read L table");
     const int Nzp = ds.Nz - 2 * ds.Yavg;
     for (int jj = 0; jj < Nzp;) {
-      const int cap = bucket_cap(hL[jj + ds.Yavg]);
+      const int sh = shape_for(plan->shapes, hL[jj + ds.Yavg]);
       int j2 = jj + 1;
-      while (j2 < Nzp && bucket_cap(hL[j2 + ds.Yavg]) == cap) ++j2;
-      plan->runs.push_back({jj, j2 - jj, cap < ds.Lmax ? cap : ds.Lmax});
+      while (j2 < Nzp && shape_for(plan->shapes, hL[j2 + ds.Yavg]) == sh) ++j2;
+      const int cap = cosk::shape_cap(sh);
+      plan->runs.push_back({jj, j2 - jj, cap < ds.Lmax ? cap : ds.Lmax, sh});
       jj = j2;
     }
     if (plan->runs.size() > 16) plan->runs.clear();
@@ -367,9 +408,118 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
     g.ds = ds;
     std::memcpy(g.ptr, ptr, sizeof(ptr));
     g.wstride = wstride;
+    g.tune_gen = gen;
     g.valid = true;
   }
   if ((e = hipGraphLaunch(g.exec, s)) != hipSuccess) return fail_hip(e, "graph launch");
+  return 0;
+}
+
+int cos_capon_autotune(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x, float* P,
+                       cos_cf32* z, cos_cf32* w, int wstride, void* stream, int* shapes_out, float* ms_out) {
+  g_last_error.clear();
+  if (!plan || !desc || !x) return fail(hipErrorInvalidValue, "plan, desc and x must be non-NULL");
+  cos_capon_desc ds = *desc;
+  if (ds.precision == COS_PREC_FP64) return fail(hipErrorInvalidValue, "the autotuner times the FP32 kernels");
+  if (ds.nsteer > 0) return fail(hipErrorInvalidValue, "the autotuner takes unsteered frames");
+  if (int e = validate(ds, nullptr, w, wstride)) return e;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  const int Nzp = ds.Nz - 2 * ds.Yavg;
+  constexpr int kRows = 128;   // output rows (all beams) timed per subarray length
+  // one run of rows per distinct L: the first rows with that length, up to kRows
+  std::map<int, std::pair<int, int>> runs;
+  if (tables) {
+    std::vector<int> hL(ds.Nz);
+    if ((e = hipMemcpyAsync(hL.data(), ds.row_L, sizeof(int) * ds.Nz, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+      return fail_hip(e, "read L table");
+    for (int jj = 0; jj < Nzp;) {
+      const int Lr = hL[jj + ds.Yavg];
+      int j2 = jj + 1;
+      while (j2 < Nzp && j2 - jj < kRows && hL[j2 + ds.Yavg] == Lr) ++j2;
+      if (Lr >= 1 && Lr <= ds.Lmax && !runs.count(Lr)) runs[Lr] = {jj, j2 - jj};
+      jj = j2;
+    }
+  } else {
+    runs[ds.L] = {0, Nzp < kRows ? Nzp : kRows};
+  }
+  const int Mmax = tables ? ds.Mmax : ds.M;
+  if (shapes_out) std::fill(shapes_out, shapes_out + 65, 0);
+  if (ms_out) std::fill(ms_out, ms_out + 65 * cosk::kShapes, -1.f);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail_hip(e, "tuning events");
+  }
+  TuneTable t{};
+  int rc = 0;
+  const char* why = nullptr;
+  for (const auto& kv : runs) {
+    const int Lr = kv.first;
+    float best_ms = 0.f;
+    for (int sh = cosk::default_shape(Lr); sh <= cosk::kShapes && !rc; ++sh) {
+      cosk::CaponLaunch a = base_launch(ds, x);
+      a.P = P;
+      a.z = (float2*)z;
+      a.w = (float2*)w;
+      a.wstride = wstride;
+      a.refine = ds.refine ? 1 : 0;
+      a.jlo = kv.second.first;
+      a.jn = kv.second.second;
+      a.npix = ds.Nx * a.jn;
+      constexpr int kReps = 3;
+      if ((e = cosk::launch_capon_f32_shape(a, sh, Mmax, s, &why)) != hipSuccess ||   // warm-up
+          (e = hipEventRecord(e0, s)) != hipSuccess) {
+        rc = why ? fail(e, why) : fail_hip(e, "tuning launch");
+        break;
+      }
+      for (int r = 0; r < kReps && e == hipSuccess; ++r) e = cosk::launch_capon_f32_shape(a, sh, Mmax, s, &why);
+      float ms = 0.f;
+      if (e != hipSuccess || (e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+          (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) {
+        rc = why ? fail(e, why) : fail_hip(e, "tuning launch");
+        break;
+      }
+      ms /= kReps;
+      if (ms_out) ms_out[Lr * cosk::kShapes + sh - 1] = ms;
+      if (!t[Lr] || ms < best_ms) {
+        t[Lr] = (unsigned char)sh;
+        best_ms = ms;
+      }
+    }
+    if (rc) break;
+    if (shapes_out) shapes_out[Lr] = t[Lr];
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneTable& dst = g_tune[tune_key(ds)];
+    for (int Lr = 1; Lr <= 64; ++Lr)
+      if (t[Lr]) dst[Lr] = t[Lr];
+  }
+  ++g_tune_gen;
+  return 0;
+}
+
+int cos_capon_tuning_set(const cos_capon_desc* desc, const int* shapes) {
+  g_last_error.clear();
+  if (!desc || !shapes) return fail(hipErrorInvalidValue, "desc and shapes must be non-NULL");
+  TuneTable t{};
+  for (int Lr = 1; Lr <= 64; ++Lr) {
+    const int sh = shapes[Lr];
+    if (sh < 0 || sh > cosk::kShapes || (sh && cosk::shape_cap(sh) < Lr))
+      return fail(hipErrorInvalidValue, "shape table entry does not hold its subarray length");
+    t[Lr] = (unsigned char)sh;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[tune_key(*desc)] = t;
+  }
+  ++g_tune_gen;
   return 0;
 }
 

@@ -18,10 +18,18 @@ static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
   return launch_one<float, G, CPL, 0, false, false, false>(a, Mmax, s);
 }
 
+hipError_t launch_capon_f32_shape(const CaponLaunch& a, int shape, int Mmax, hipStream_t s, const char** why) {
+  switch (shape) {
+    case 1: return launch_f32_tm<16, 1>(a, Mmax, s);
+    case 2: return launch_f32_tm<16, 2>(a, Mmax, s);
+    case 3: return launch_f32_tm<64, 1>(a, Mmax, s);
+  }
+  if (why) *why = "unknown lane shape";
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why) {
-  if (Lmax <= 16) return launch_f32_tm<16, 1>(a, Mmax, s);
-  if (Lmax <= 32) return launch_f32_tm<16, 2>(a, Mmax, s);
-  if (Lmax <= 64) return launch_f32_tm<64, 1>(a, Mmax, s);
+  if (Lmax <= 64) return launch_capon_f32_shape(a, default_shape(Lmax), Mmax, s, why);
   if (why) *why = "L > 64 is not supported by the fused kernel";
   return hipErrorInvalidValue;
 }

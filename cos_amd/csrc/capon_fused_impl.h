@@ -626,6 +626,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         // only meets a zero (y_l for l >= L, beta_k for k >= K) or a padding column,
         // whose residual is dropped.
         const float2* srcw = a.x + ((size_t)ix * a.Nz + j) * a.Mstride + off;
+        if constexpr (G == 64) rowb[g] = y[0];   // the row buffers are dead after the factorisation
         cplx<T> sy[CPL];
         static_for<0, CPL>([&](auto S_) {
           constexpr int S = decltype(S_)::value;
@@ -654,30 +655,76 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
               }
               break;
             }
-            // 64-lane groups: beta needs l < L, the column terms l2 < K - kb (uniform)
-            const int nk = SKIP ? __builtin_amdgcn_readfirstlane(K - kb) : G;
-            cplx<T> xv[LMAX];
-            static_for<0, LMAX>([&](auto l_) {
-              constexpr int l = decltype(l_)::value;
-              if (SKIP && l >= Lw && l >= nk) return;
-              xv[l] = ldx<T>(xt + min(kb + g + l, M - 1));
-            });
-            cplx<T> nb = {T(0), T(0)};   // -(y^H u_{t,k}) = -conj(beta_{t,k})
-            static_for<0, LMAX>([&](auto l_) {
-              constexpr int l = decltype(l_)::value;
-              if (SKIP && l >= Lw) return;
-              bcast_sub_conj_mul<G, l % G>(nb, y[l / G], xv[l]);
-            });
-            if (kb + g >= K) nb = {T(0), T(0)};
-            // (S y)_c += x_t[k + c] beta_{t,k} for the G subarrays of the block
-            static_for<0, G>([&](auto l2_) {
-              constexpr int l2 = decltype(l2_)::value;
-              if (SKIP && l2 >= nk) return;   // k = kb + l2 >= K: beta_k = 0
-              if constexpr (CPL == 2)
-                bcast_sub_conj_mul2<G, l2>(sy[0], sy[1], nb, xv[l2], xv[l2 + G]);
-              else
-                bcast_sub_conj_mul<G, l2>(sy[0], nb, xv[l2]);
-            });
+            if constexpr (G == 64) {
+              // one pixel per wave: y_l from LDS (one broadcast read each, nothing held in
+              // scalar registers) and the window in two halves of 32, the low half loaded
+              // again for the column terms, so the refinement stays within the
+              // factorisation's register budget.  beta needs l < L, the column terms
+              // l2 < K - kb (both uniform).
+              const int nk = __builtin_amdgcn_readfirstlane(K - kb);
+              // an offset the compiler cannot see through keeps the y reads inside the
+              // loop (hoisted, the 64 values would hold 128 VGPRs across it)
+              int yo = 0;
+              asm volatile("" : "+v"(yo));
+              const cplx<T>* const yl = rowb + yo;
+              cplx<T> xv[32];
+              cplx<T> nb = {T(0), T(0)};
+              auto load_half = [&](int h) {
+                static_for<0, 32>([&](auto l_) {
+                  constexpr int l = decltype(l_)::value;
+                  if (h + l < Lw || h + l < nk) xv[l] = ldx<T>(xt + min(kb + g + h + l, M - 1));
+                });
+              };
+              auto beta_half = [&](int h) {
+                static_for<0, 32>([&](auto l_) {
+                  constexpr int l = decltype(l_)::value;
+                  if (h + l < Lw) sub_conj_mul(nb, yl[h + l], xv[l]);
+                });
+              };
+              // beta_k reaches the column owners through LDS as well (broadcast reads)
+              const cplx<T>* const bl = rowb + LMAX + yo;
+              auto cols_half = [&](auto H_) {
+                constexpr int h = decltype(H_)::value;
+                static_for<0, 32>([&](auto l_) {
+                  constexpr int l2 = decltype(l_)::value;
+                  if (h + l2 < nk) sub_conj_mul(sy[0], bl[h + l2], xv[l2]);
+                });
+              };
+              load_half(0);
+              beta_half(0);
+              const bool two = Lw > 32 || nk > 32;
+              if (two) {
+                load_half(32);
+                beta_half(32);
+              }
+              if (kb + g >= K) nb = {T(0), T(0)};
+              rowb[LMAX + g] = nb;
+              if (two) {
+                cols_half(ic<32>{});
+                load_half(0);
+              }
+              cols_half(ic<0>{});
+            } else {
+              cplx<T> xv[LMAX];
+              static_for<0, LMAX>([&](auto l_) {
+                constexpr int l = decltype(l_)::value;
+                xv[l] = ldx<T>(xt + min(kb + g + l, M - 1));
+              });
+              cplx<T> nb = {T(0), T(0)};   // -(y^H u_{t,k}) = -conj(beta_{t,k})
+              static_for<0, LMAX>([&](auto l_) {
+                constexpr int l = decltype(l_)::value;
+                bcast_sub_conj_mul<G, l % G>(nb, y[l / G], xv[l]);
+              });
+              if (kb + g >= K) nb = {T(0), T(0)};
+              // (S y)_c += x_t[k + c] beta_{t,k} for the G subarrays of the block
+              static_for<0, G>([&](auto l2_) {
+                constexpr int l2 = decltype(l2_)::value;
+                if constexpr (CPL == 2)
+                  bcast_sub_conj_mul2<G, l2>(sy[0], sy[1], nb, xv[l2], xv[l2 + G]);
+                else
+                  bcast_sub_conj_mul<G, l2>(sy[0], nb, xv[l2]);
+              });
+            }
           }
         }
         cplx<T> r[CPL];
@@ -892,7 +939,7 @@ __device__ __noinline__ void capon_group_call(const CaponLaunch& a, const int MP
 
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
-                          amdgpu_waves_per_eu(BWD ? 1 : OccHint<T, G, CPL, TM>::value)))
+                          amdgpu_waves_per_eu(BWD ? 1 : (REF && G == 64) ? 2 : OccHint<T, G, CPL, TM>::value)))
 void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   // PERSIST: grid-stride over groups of NG pixels, so a launch over a device-side pixel
   // list (the FP64 fallback) can use a bounded grid; otherwise one block per group (a

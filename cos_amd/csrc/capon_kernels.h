@@ -44,8 +44,16 @@ struct CaponLaunch {
   int fb, zmode, pmode;
 };
 
+// Lane shapes of the FP32 fused kernel, the autotuner's choices per subarray length:
+// 1 = 16 lanes x 1 column (L <= 16), 2 = 16 lanes x 2 columns (L <= 32), 3 = 64 lanes
+// (L <= 64).  default_shape is the smallest shape that holds L.
+constexpr int kShapes = 3;
+inline int shape_cap(int shape) { return shape == 1 ? 16 : shape == 2 ? 32 : shape == 3 ? 64 : 0; }
+inline int default_shape(int L) { return L <= 16 ? 1 : L <= 32 ? 2 : 3; }
+
 // Returns hipSuccess or an error; `why` receives a static message on argument errors.
 hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
+hipError_t launch_capon_f32_shape(const CaponLaunch& a, int shape, int Mmax, hipStream_t s, const char** why);
 hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
 hipError_t launch_capon_bwd(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
 

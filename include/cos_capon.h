@@ -128,6 +128,24 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
                   float* P, cos_cf32* z, cos_cf32* w, int wstride, cos_cf32* R,
                   unsigned char* flags, void* stream);
 
+/* Autotuner of the fused FP32 kernel's lane shape (no reference counterpart; the
+ * reference has one kernel configuration).  For each subarray length the frame uses (the
+ * uniform L, or each distinct value of the row table) it times every lane shape that holds
+ * that length (1: 16 lanes x 1 column, L <= 16; 2: 16 lanes x 2 columns, L <= 32;
+ * 3: 64 lanes, L <= 64) on up to 128 output rows of that length (all beams) and keeps the
+ * fastest for this device and frame configuration (Mmax, Yavg, options, refine); later
+ * cos_capon_run calls on the device use it.  P, z, w (device, nullable, as cos_capon_run)
+ * receive the timed rows' results.  shapes_out [65] (nullable): chosen shape per L, 0 where
+ * the frame has no row of that length; ms_out [65*3] (nullable): ms per launch of shape s
+ * at L in ms_out[3*L + s - 1], negative where not timed.  Synchronous on stream. */
+int cos_capon_autotune(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x, float* P,
+                       cos_cf32* z, cos_cf32* w, int wstride, void* stream, int* shapes_out, float* ms_out);
+
+/* Installs a lane-shape table shapes[65] (0 = the smallest shape that holds L) for the
+ * current device and the frame configuration of desc, e.g. one saved from
+ * cos_capon_autotune. */
+int cos_capon_tuning_set(const cos_capon_desc* desc, const int* shapes);
+
 /* Beamspace Capon with the sliding DFT: replaces the reference's SLIDING_BEAMSPACE build
  * (RealTimeCapon/sliding_beamspace_kernel.cu:26-237 -> build_R_full -> Nb-solve ->
  * getCaponOutputFromSubarraySums, RealTimeCapon/Capon.cpp:202-227,399-415) with loading

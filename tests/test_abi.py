@@ -78,9 +78,30 @@ def test_plan_run_rejects_bad_descriptor():
     assert L.cos_capon_plan_destroy(h) == 0
 
 
+def test_tuning_table_rejects_shapes_too_small_for_L():
+    from cos_amd.capon import CaponDesc
+    L = _lib.lib()
+    ds = CaponDesc(Nx=1, Nz=8, Mstride=64, M=64, L=32, off=0, Yavg=1, d=0.1)
+    tab = (C.c_int * 65)()
+    tab[20] = 1          # 16 lanes x 1 column holds L <= 16 only
+    code, text = _err(L.cos_capon_tuning_set(C.byref(ds), tab))
+    assert code != 0 and "does not hold its subarray length" in text
+    tab[20] = 4          # no such shape
+    code, text = _err(L.cos_capon_tuning_set(C.byref(ds), tab))
+    assert code != 0
+
+
 def test_getCaponCUDAPy_returns_error_code_like_reference():
     from cos_amd import getCaponCUDAPy
     x = np.zeros((2, 5, 4), np.complex64)
     z, R, w, e = getCaponCUDAPy(x, 0.1, 6, 1, np.array([0]), False, False)   # L > M
     assert e != 0
     assert z.shape == (2, 3) and w.shape == (2, 3, 6) and R.shape == (2, 3, 6, 6)
+
+
+def test_every_export_has_argtypes():
+    # ctypes passes a Python int without argtypes as a 32-bit int: pointers and stream
+    # handles would be truncated
+    L = _lib.lib()
+    for name in _lib.EXPORTS:
+        assert getattr(L, name).argtypes is not None, name

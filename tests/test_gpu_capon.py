@@ -203,6 +203,66 @@ def test_refined_rows_tables_with_fallback():
     assert rel(o["P"][flagged], ref["P"][flagged]).max() <= 1.2e-7
 
 
+# ------------------------------------------------------------------ lane-shape autotuner
+
+@pytest.mark.parametrize("M,L", [(24, 12), (64, 32)])
+def test_every_lane_shape_matches_oracle(M, L):
+    """Each lane shape that holds L, forced through the tuning table, against the FP64
+    oracle; the table is reset afterwards."""
+    import torch
+    from cos_amd import CaponPlan
+    x = uniform_cube(5 * M + L, 3, 20, M)
+    xt = torch.from_numpy(x).cuda()
+    ref = O.capon(x.astype(np.complex128), 0.05, L, 2)
+    try:
+        for sh in (1, 2, 3):
+            if (16, 32, 64)[sh - 1] < L:
+                continue
+            say("forced shape", M, L, sh)
+            CaponPlan.set_tuning(xt, 0.05, L, 2, {L: sh})
+            o = CaponPlan().run(xt, 0.05, L, 2, precision="fp32+fallback")
+            torch.cuda.synchronize()
+            assert rel(o["P"].cpu().numpy(), ref["P"].ravel()).max() <= 1e-4
+            np.testing.assert_allclose(o["w"].cpu().numpy(), ref["w"].reshape(-1, L),
+                                       atol=1e-3 * np.abs(ref["w"]).max())
+    finally:
+        CaponPlan.set_tuning(xt, 0.05, L, 2, {})
+
+
+def test_autotune_aperture_rows():
+    """The autotuner over an expanding-aperture frame (L 8..64): it times every shape that
+    holds each length, installs the fastest, and the tuned frame still matches FP64; a
+    captured graph is re-captured after the table changes."""
+    import torch
+    from cos_amd import CaponPlan, synth
+    say("autotune aperture", 2, 1024, 128)
+    Nx, Nz, Y = 2, 1024, 2
+    Mz, Lz, off, dz = synth.config3_geometry(Nz, seed=3, Mmax=128)
+    x = synth.speckle_cube(Nx, Nz, 128, seed=3)
+    xt = torch.from_numpy(x).cuda()
+    rows = {k: torch.from_numpy(v).cuda() for k, v in dict(L=Lz, M=Mz, off=off, d=dz).items()}
+    plan = CaponPlan()
+    try:
+        out = plan.run(xt, 0.0, 0, Y, rows=rows, precision="fp32+fallback", graph=True)
+        tuned = plan.autotune(xt, 0.0, 0, Y, rows=rows)
+        Ls = sorted(set(Lz[Y:Nz - Y].tolist()))
+        assert sorted(tuned["shapes"]) == Ls
+        for l, sh in tuned["shapes"].items():
+            valid = [s for s in (1, 2, 3) if (16, 32, 64)[s - 1] >= l]
+            assert sorted(tuned["ms"][l]) == valid
+            assert tuned["ms"][l][sh] == min(tuned["ms"][l].values())
+        print("tuned shapes", tuned["shapes"], flush=True)
+        o = plan.run(xt, 0.0, 0, Y, rows=rows, precision="fp32+fallback", graph=True, out=out)
+        torch.cuda.synchronize()
+        Nzp = Nz - 2 * Y
+        sel = np.arange(0, Nx * Nzp, 7)
+        ref = O.capon_rows(x.astype(np.complex128), dz.astype(np.float64), Lz, Mz, off, Y,
+                           pixels=(sel // Nzp, sel % Nzp))
+        assert rel(o["P"].cpu().numpy()[sel], ref["P"]).max() <= 1e-4
+    finally:
+        CaponPlan.set_tuning(xt, 0.0, 0, Y, {}, rows=rows)
+
+
 # ------------------------------------------------------------------ per-row apertures and fallback
 
 def test_rows_tables_and_fp64_fallback():
