@@ -132,9 +132,10 @@ def main():
     }
     stream = torch.cuda.current_stream(dev)
 
-    def step(prec="fp32+fallback", graph=True, refine=False):
+    def step(prec="fp32+fallback", graph=True, refine=False, cov_mfma=False):
         # one frame; the launches are captured once in a hipGraph and replayed
-        plan.run(x, D, L, Y, precision=prec, out=out, stream=stream.cuda_stream, graph=graph, refine=refine)
+        plan.run(x, D, L, Y, precision=prec, out=out, stream=stream.cuda_stream, graph=graph, refine=refine,
+                 cov_mfma=cov_mfma)
 
     def timed(fn, n):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -176,6 +177,14 @@ def main():
     res["refined"] = {"ms_per_frame": round(ref_ms, 3), "pixels_per_s": NPX / (ref_ms * 1e-3),
                       "P_rel_err_max": float(err_r.max()), "P_rel_err_mean": float(err_r.mean()),
                       "unrefined_P_rel_err_mean": float(err.mean())}
+    # the covariance on the matrix cores (BF16 hi/lo split, FP32 accumulation) into a plan
+    # buffer, then the FP32 solve from it
+    mf_ms = timed(lambda: step(cov_mfma=True, graph=False), max(3, args.steps // 4))
+    Pm = out["P"].cpu().numpy().reshape(NX, NZ - 2 * Y)
+    err_m = np.abs(Pm[:, 900:916] - ref["P"]) / ref["P"]
+    res["mfma_covariance"] = {"ms_per_frame": round(mf_ms, 3), "pixels_per_s": NPX / (mf_ms * 1e-3),
+                              "P_rel_err_max": float(err_m.max()), "P_rel_err_mean": float(err_m.mean()),
+                              "path": "BF16 hi/lo MFMA covariance -> R in HBM -> FP32 Cholesky solve"}
 
     line = {
         "metric": "Capon pixels/sec at M=64,L=32 (<=1e-4 rel err vs FP64); % HBM/MFMA roofline",

@@ -34,7 +34,7 @@ class CaponDesc(C.Structure):
         ("Lmax", C.c_int), ("Mmax", C.c_int),
         ("precision", C.c_int), ("cond_thresh", C.c_float),
         ("steer_sin", C.c_void_p), ("nsteer", C.c_int), ("graph", C.c_int),
-        ("fb_avg", C.c_int), ("z_mode", C.c_int), ("p_mode", C.c_int), ("refine", C.c_int),
+        ("fb_avg", C.c_int), ("z_mode", C.c_int), ("p_mode", C.c_int), ("refine", C.c_int), ("cov_mfma", C.c_int),
     ]
 
 

@@ -142,7 +142,7 @@ class CaponPlan:
 
     def run(self, x, d, L, Yavg, *, M=None, off=0, precision="fp32", want=("P", "z", "w"),
             rows=None, steer_sin=None, cond_thresh=1e6, out=None, stream=None, graph=False,
-            fb_avg=False, z_mode="amplitude", p_mode="loaded", refine=False):
+            fb_avg=False, z_mode="amplitude", p_mode="loaded", refine=False, cov_mfma=False):
         """Run one frame.
 
         x: complex64 CUDA tensor [Nx][Nz][Mstride].  rows: optional dict of CUDA tensors
@@ -153,13 +153,15 @@ class CaponPlan:
         (pass the same ``out`` dict to keep the pointers).  fb_avg: forward-backward
         averaged covariance; z_mode: "amplitude" (w^H g) or "power" (power Capon);
         p_mode: "loaded" (1/(a^H R_L^-1 a)) or "unloaded" (|w^H R_hat w|).  refine: one step
-        of iterative refinement on the FP32 paths (residual from the snapshots).  Returns a
-        dict of output tensors.
+        of iterative refinement on the FP32 paths (residual from the snapshots).  cov_mfma:
+        covariance on the matrix cores (BF16 hi/lo split), then the FP32 solve from it
+        (uniform L <= 32, default options).  Returns a dict of output tensors.
         """
         ds, o, ptr, wstride, stream = self._prepare(
             x, d, L, Yavg, M=M, off=off, precision=precision, want=want, rows=rows, steer_sin=steer_sin,
             cond_thresh=cond_thresh, out=out, stream=stream, graph=graph, fb_avg=fb_avg, z_mode=z_mode,
             p_mode=p_mode, refine=refine)
+        ds.cov_mfma = int(bool(cov_mfma))
         check(lib().cos_capon_run(self._h, C.byref(ds), x.data_ptr(), ptr("P"), ptr("z"), ptr("w"),
                                   wstride, ptr("R"), ptr("flags"), stream))
         return o

@@ -12,7 +12,7 @@ static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
   // one step of iterative refinement: its own runtime-window kernels (more LDS)
   if (a.refine) return launch_one<float, G, CPL, 0, false, false, 0, true>(a, Mmax, s);
   // the optional variants live in the runtime-window kernels
-  if (a.fb || a.zmode || a.pmode) return launch_one<float, G, CPL, 0, false, false, 0>(a, Mmax, s);
+  if (a.fb || a.zmode || a.pmode || a.Rin) return launch_one<float, G, CPL, 0, false, false, 0>(a, Mmax, s);
   if (TW <= 5) return launch_one<float, G, CPL, 5, false, false, false>(a, Mmax, s);
   if (TW <= 9) return launch_one<float, G, CPL, 9, false, false, false>(a, Mmax, s);
   return launch_one<float, G, CPL, 0, false, false, false>(a, Mmax, s);

@@ -168,7 +168,21 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   cplx<T> A[NS];
 
   // ---------------- covariance S (unnormalised) ----------------
-  if constexpr (DIRECT) {
+  // (or the loaded S_L of every pixel given in a.Rin, runtime-window FP32 kernels only)
+  constexpr bool FR = OPTS && !REF && std::is_same<T, float>::value;
+  bool from_r = false;
+  if constexpr (FR) from_r = a.Rin != nullptr;
+  if (from_r) {
+    const float2* Rp = a.Rin + (size_t)p * L * L;
+    static_for<0, CPL>([&](auto S_) {
+      constexpr int S = decltype(S_)::value;
+      static_for<0, G * (S + 1)>([&](auto R_) {
+        constexpr int R = decltype(R_)::value;
+        const float2 v = (R < L && cs[S] < L) ? Rp[R * L + cs[S]] : make_float2(0.f, 0.f);
+        A[slot<G, S, R>()] = {(T)v.x, (T)-v.y};
+      });
+    });
+  } else if constexpr (DIRECT) {
     static_for<0, NS>([&](auto I_) {
     constexpr int I = decltype(I_)::value; A[I] = {T(0), T(0)}; });
     for (int t = 0; t < TW; ++t) {
@@ -328,7 +342,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     }
   }
   const T trS = Gp::sum(trp);
-  const T delta = (T)dload * trS / (T)L;
+  const T delta = from_r ? T(0) : (T)dload * trS / (T)L;   // a.Rin is loaded already
   const T invK = T(1) / (T)K;
   const bool steered = a.nsteer > 0;
   // subarray mean of the centre snapshot, g_c = (1/K) sum_k x_c[k+c], for this lane's

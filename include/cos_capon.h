@@ -103,6 +103,9 @@ typedef struct {
     int refine;          /* nonzero: one step of iterative refinement on the FP32 paths, the
                             residual a - R_L y taken from the snapshots themselves (no
                             steering, no forward-backward averaging) */
+    int cov_mfma;        /* nonzero: covariance on the matrix cores (BF16 hi/lo split, FP32
+                            accumulation) into R (or a plan buffer when R is NULL), then the
+                            FP32 solve from it; uniform L <= 32, default options */
 } cos_capon_desc;
 
 enum {

@@ -75,6 +75,13 @@ def test_plan_run_rejects_bad_descriptor():
         ds = CaponDesc(Nx=1, Nz=8, Mstride=8, M=8, L=4, off=0, Yavg=1, d=0.1, refine=1, **kw)
         code, text = _err(L.cos_capon_run(h, C.byref(ds), 16, None, None, None, 0, None, None, None))
         assert code != 0 and "refinement applies to the FP32 paths" in text
+    # the matrix-core covariance covers uniform L <= 32 with the default options
+    for kw in (dict(L=40), dict(refine=1), dict(p_mode=1)):
+        args = dict(Nx=1, Nz=8, Mstride=64, M=64, L=16, off=0, Yavg=1, d=0.1, cov_mfma=1)
+        args.update(kw)
+        ds = CaponDesc(**args)
+        code, text = _err(L.cos_capon_run(h, C.byref(ds), 16, None, None, None, 0, None, None, None))
+        assert code != 0, kw
     assert L.cos_capon_plan_destroy(h) == 0
 
 

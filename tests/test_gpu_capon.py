@@ -203,6 +203,38 @@ def test_refined_rows_tables_with_fallback():
     assert rel(o["P"][flagged], ref["P"][flagged]).max() <= 1.2e-7
 
 
+# ------------------------------------------------------------------ matrix-core covariance
+
+@pytest.mark.parametrize("M,L,Y", [(64, 32, 2), (24, 12, 1), (40, 17, 0), (16, 1, 2)])
+def test_mfma_covariance_matches_oracle(M, L, Y):
+    """The covariance on the matrix cores (BF16 hi/lo split, FP32 accumulation) in the
+    reference R layout (loaded, unnormalised, conj-stored) against the FP64 oracle: within
+    2e-5 of the largest entry (three BF16 products keep 16 significant bits of each
+    operand; measured 3-4e-6); the solve from it gives P within 2e-4 of FP64."""
+    say("mfma covariance", M, L, Y)
+    x = uniform_cube(11 * M + L, 3, 10, M)
+    o, _ = _plan_run(x, 0.05, L, Y, cov_mfma=True, want=("P", "z", "w", "R"))
+    ref = O.reference_gpu_outputs(x.astype(np.complex128), float(np.float32(0.05)), L, Y)
+    Rref = ref["R"].reshape(o["R"].shape)
+    eR = np.abs(o["R"] - Rref).max() / np.abs(Rref).max()
+    refP = O.capon(x.astype(np.complex128), 0.05, L, Y)
+    eP = rel(o["P"], refP["P"].ravel())
+    print(f"R err {eR:.2e} (of max), P err max {eP.max():.2e}", flush=True)
+    assert eR <= 2e-5
+    assert eP.max() <= 2e-4
+
+
+def test_mfma_covariance_cfg2_slice(cfg2_slice):
+    """The 64-element, L = 32 slice through the matrix-core covariance: power within 1e-4
+    of FP64 (measured max 5e-5, mean 5e-6)."""
+    say("cfg2 slice mfma covariance", 8, 2048, 64, 32)
+    x, ref = cfg2_slice
+    o, _ = _plan_run(x, 0.01, 32, 2, cov_mfma=True)
+    e = rel(o["P"], ref["P"].ravel())
+    print(f"mfma-covariance P err max {e.max():.3e} mean {e.mean():.3e}", flush=True)
+    assert e.max() <= 1e-4
+
+
 # ------------------------------------------------------------------ lane-shape autotuner
 
 @pytest.mark.parametrize("M,L", [(24, 12), (64, 32)])
