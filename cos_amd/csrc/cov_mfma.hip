@@ -15,40 +15,22 @@
 // products P Q with the same Q[n'][j] = (Re, Im)(A[n][j]) and P[i][n'] = Q[n'][i] for the
 // real part, P[i][n'] = (Im, -Re)(A[n][i]) for the imaginary part.
 //
-// v_mfma_f32_32x32x8_bf16 layouts (wave64): lane l supplies A[i = l%32][k = 4(l/32) + q]
-// and B[k = 4(l/32) + q][j = l%32], q < 4, and holds D[i][j] for j = l%32,
+// v_mfma_f32_32x32x16_bf16 layouts (wave64): lane l supplies A[i = l%32][k = 8(l/32) + q]
+// and B[k = 8(l/32) + q][j = l%32], q < 8, and holds D[i][j] for j = l%32,
 // i = 8(v/4) + 4(l/32) + v%4 in accumulator element v < 16.  For one MFMA step s the lane
-// therefore needs the two complex values A[n][l%32] at n = 4s + 2(l/32) and n + 1, for P
-// and Q alike: each lane reads its own subarray element twice per step and nothing is
-// exchanged between lanes.
+// therefore needs the four complex values A[n][l%32], n = 8s + 4(l/32) + 0..3, for P and
+// Q alike: each lane reads its own subarray elements and nothing is exchanged between
+// lanes.  The split uses the hardware conversion (v_cvt_pk_bf16_f32, round to nearest
+// even); the imaginary part's P is the real part's operands with (Re, Im) -> (Im, -Re),
+// a swap and a sign flip of the BF16 bits.
 #include "cos_device.h"
 #include "capon_kernels.h"
 
 namespace cosk {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-// FP32 -> BF16, round to nearest even (inputs are finite)
-__device__ __forceinline__ unsigned bf16_bits(float f) {
-  const unsigned u = __float_as_uint(f);
-  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
-
-struct HiLo {
-  bf16x4 hi, lo;
-};
-__device__ __forceinline__ HiLo split4(float a, float b, float c, float d) {
-  const float v[4] = {a, b, c, d};
-  HiLo s;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const unsigned h = bf16_bits(v[q]);
-    s.hi[q] = (short)h;
-    s.lo[q] = (short)bf16_bits(v[q] - __uint_as_float(h << 16));
-  }
-  return s;
-}
 
 // One pixel per wave (block of 64).  The window's 2Y+1 rows are staged in LDS, zero padded
 // to MP elements (MP >= M + 32, so every subarray element of a padding row or column reads
@@ -66,51 +48,77 @@ __global__ __launch_bounds__(64) void cov_mfma_kernel(const float2* __restrict__
     for (int m = lane; m < MP; m += 64) rows[t * MP + m] = m < M ? src[(size_t)t * Mstride + m] : make_float2(0.f, 0.f);
   __syncthreads();
 
-  const int c = lane & 31, h = lane >> 5;
-  // A[n][c] = x_t[k + c] with n = t K + k, zero for n >= N
-  auto elem = [&](int n) -> float2 {
-    if (n >= N) return make_float2(0.f, 0.f);
+  // offset of subarray n = t K + k in the staged rows, t MP + k; past the last subarray
+  // (up to a whole number of steps) the zero tail of row 0 (offset M, + lane < MP)
+  int* const offs = reinterpret_cast<int*>(rows + TW * MP);
+  const int NS = (N + 7) / 8;   // MFMA steps of 8 subarrays (16 real inner indices)
+  for (int n = lane; n < 8 * NS; n += 64) {
     const int t = n / K;
-    return rows[t * MP + (n - t * K) + c];
-  };
-  f32x16 sr = {}, si = {};
-  // the same trip count for both half-waves: every MFMA runs with the full wave
-  const int NS = (N + 3) / 4;
-  for (int s4 = 0; s4 < NS; ++s4) {
-    const int n0 = 4 * s4 + 2 * h;
-    const float2 v0 = elem(n0), v1 = elem(n0 + 1);
-    const HiLo q = split4(v0.x, v0.y, v1.x, v1.y);     // Q, and P of the real part
-    const HiLo pi = split4(v0.y, -v0.x, v1.y, -v1.x);  // P of the imaginary part
-    sr = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(q.hi, q.hi, sr, 0, 0, 0);
-    sr = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(q.hi, q.lo, sr, 0, 0, 0);
-    sr = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(q.lo, q.hi, sr, 0, 0, 0);
-    si = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(pi.hi, q.hi, si, 0, 0, 0);
-    si = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(pi.hi, q.lo, si, 0, 0, 0);
-    si = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(pi.lo, q.hi, si, 0, 0, 0);
+    offs[n] = n < N ? t * MP + (n - t * K) : M;
   }
+  __syncthreads();
+
+  const int c = lane & 31, h = lane >> 5;
+  // four independent accumulators: hi*hi and the two cross products, real and imaginary
+  f32x16 srh = {}, srx = {}, sih = {}, six = {};
+  // the same trip count for both half-waves: every MFMA runs with the full wave
+  for (int s8 = 0; s8 < NS; ++s8) {
+    float v[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float2 e = rows[offs[8 * s8 + 4 * h + q] + c];
+      v[2 * q] = e.x;
+      v[2 * q + 1] = e.y;
+    }
+    bf16x8 qh, ql;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      qh[q] = (__bf16)v[q];
+      ql[q] = (__bf16)(v[q] - (float)qh[q]);
+    }
+    // (Re, Im) -> (Im, -Re) on the BF16 bits
+    const u16x8 hb = __builtin_bit_cast(u16x8, qh), lb = __builtin_bit_cast(u16x8, ql);
+    u16x8 ph, pl;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ph[2 * q] = hb[2 * q + 1];
+      ph[2 * q + 1] = hb[2 * q] ^ 0x8000;
+      pl[2 * q] = lb[2 * q + 1];
+      pl[2 * q + 1] = lb[2 * q] ^ 0x8000;
+    }
+    const bf16x8 pih = __builtin_bit_cast(bf16x8, ph), pil = __builtin_bit_cast(bf16x8, pl);
+    srh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qh, qh, srh, 0, 0, 0);
+    sih = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih, qh, sih, 0, 0, 0);
+    srx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qh, ql, srx, 0, 0, 0);
+    six = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih, ql, six, 0, 0, 0);
+    srx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ql, qh, srx, 0, 0, 0);
+    six = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pil, qh, six, 0, 0, 0);
+  }
+  const f32x16 sr = srh + srx, si = sih + six;
 
   // tr(S) = sum_t sum_m |x_t[m]|^2 #{(k, l): k + l = m, k < K, l < L}, from the data (FP32)
   float trp = 0.f;
   for (int t = 0; t < TW; ++t)
     for (int m = lane; m < M; m += 64) {
-      const float2 v = rows[t * MP + m];
+      const float2 e = rows[t * MP + m];
       const int cnt = min(L - 1, m) - max(0, m - K + 1) + 1;
-      trp = fmaf((float)cnt, fmaf(v.x, v.x, v.y * v.y), trp);
+      trp = fmaf((float)cnt, fmaf(e.x, e.x, e.y * e.y), trp);
     }
   const float delta = d * Grp<64>::sum(trp) / (float)L;
 
   float2* Rp = R + (size_t)p * L * L;
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int i = 8 * (v / 4) + 4 * h + (v % 4);
-    if (i < L && c < L) Rp[i * L + c] = make_float2(sr[v] + (i == c ? delta : 0.f), -si[v]);
+  for (int vv = 0; vv < 16; ++vv) {
+    const int i = 8 * (vv / 4) + 4 * h + (vv % 4);
+    if (i < L && c < L) Rp[i * L + c] = make_float2(sr[vv] + (i == c ? delta : 0.f), -si[vv]);
   }
 }
 
 hipError_t launch_cov_mfma(const CaponLaunch& a, hipStream_t s) {
   if (a.rowL || a.plist || a.L < 1 || a.L > 32 || a.M < a.L || !a.R) return hipErrorInvalidValue;
   const int MP = ((a.M + 32 + 3) / 4) * 4;
-  const size_t lds = (size_t)(2 * a.Y + 1) * MP * sizeof(float2);
+  const int N = (2 * a.Y + 1) * (a.M - a.L + 1);
+  const size_t lds = (size_t)(2 * a.Y + 1) * MP * sizeof(float2) + (size_t)((N + 7) / 8 * 8) * sizeof(int);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   const int npix = a.Nx * (a.Nz - 2 * a.Y);
   if (npix == 0) return hipSuccess;
