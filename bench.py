@@ -103,7 +103,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparator")
     ap.add_argument("--workload", default="frame",
-                    choices=["frame", "aperture", "beamspace", "steering", "adjoint", "tdib", "lca"],
+                    choices=["frame", "aperture", "beamspace", "steering", "adjoint", "tdib", "lca", "shard"],
                     help="frame: the headline 256x2048x64 L=32 frame (default); the others time the "
                          "expanding-aperture, sonar beamspace, steered and differentiable workloads")
     args = ap.parse_args()
@@ -162,9 +162,11 @@ def main():
     # accuracy on a sample vs the FP64 oracle, and the plain FP64 GPU kernel
     import oracle
     P = out["P"].cpu().numpy().reshape(NX, NZ - 2 * Y)
-    ref = oracle.c_capon(x_np, D, L, Y, ix=(0, NX, ), j=(900, 916), want_w=False,
+    Wt = out["w"].cpu().numpy().reshape(NX, NZ - 2 * Y, L)[:, 900:916]
+    ref = oracle.c_capon(x_np, D, L, Y, ix=(0, NX, ), j=(900, 916), want_w=True,
                          nthreads=os.cpu_count() or 1)
     err = np.abs(P[:, 900:916] - ref["P"]) / ref["P"]
+    werr = np.linalg.norm(Wt - ref["w"], axis=-1) / np.linalg.norm(ref["w"], axis=-1)
     res = {}
     if not args.no_fp64:
         fp64_ms = timed(lambda: step("fp64", graph=False), 2)
@@ -207,7 +209,8 @@ def main():
                      "traffic": measured_traffic(), "kernel": "capon_kernel<float,16,2,5,false,false,0>",
                      "kernel_ms": round(kern_ms, 4), "flop_per_pixel": FLOP_PER_PX},
         "accuracy": {"P_rel_err_max": float(err.max()), "P_rel_err_p999": float(np.quantile(err, 0.999)),
-                     "sample": "256 beams x rows 900..915 vs FP64 oracle"},
+                     "w_rel_err_max": float(werr.max()), "w_rel_err_mean": float(werr.mean()),
+                     "sample": "256 beams x rows 900..915 vs FP64 oracle (w: ||w - w64|| / ||w64||)"},
         "wall_s_timed": round(wall, 3),
         "input_gen_s": round(gen_s, 2),
     }

@@ -253,8 +253,8 @@ int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
   if (ds.p_mode != COS_P_LOADED && ds.p_mode != COS_P_UNLOADED) return fail(hipErrorInvalidValue, "bad p_mode");
   if (ds.nsteer > 0 && (ds.z_mode != COS_Z_AMPLITUDE || ds.p_mode != COS_P_LOADED))
     return fail(hipErrorInvalidValue, "steering takes the default z and P definitions");
-  if (ds.refine && (ds.precision == COS_PREC_FP64 || ds.nsteer > 0 || ds.fb_avg))
-    return fail(hipErrorInvalidValue, "refinement applies to the FP32 paths without steering or averaging");
+  if (ds.refine && (ds.precision == COS_PREC_FP64 || ds.fb_avg))
+    return fail(hipErrorInvalidValue, "refinement applies to the FP32 paths without forward-backward averaging");
   if (ds.cov_mfma && (tables || ds.L > 32 || ds.precision == COS_PREC_FP64 || ds.nsteer > 0 || ds.fb_avg ||
                       ds.z_mode || ds.p_mode || ds.refine))
     return fail(hipErrorInvalidValue, "the MFMA covariance takes a uniform L <= 32, an FP32 path and default options");

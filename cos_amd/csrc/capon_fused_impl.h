@@ -741,10 +741,21 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
             }
           }
         }
+        // the right-hand side a: all ones, or the steering vector exp(j pi c sin(theta))
+        cplx<T> av[CPL];
+        static_for<0, CPL>([&](auto S_) {
+          constexpr int S = decltype(S_)::value;
+          av[S] = {T(1), T(0)};
+          if (steered) {
+            float sv, cv;
+            sincospif((float)(sn * (T)cs[S]), &sv, &cv);
+            av[S] = {(T)cv, (T)sv};
+          }
+        });
         cplx<T> r[CPL];
         static_for<0, CPL>([&](auto S_) {
           constexpr int S = decltype(S_)::value;
-          r[S] = cs[S] < L ? cplx<T>{T(1) - sy[S].x - delta * y[S].x, -sy[S].y - delta * y[S].y}
+          r[S] = cs[S] < L ? cplx<T>{av[S].x - sy[S].x - delta * y[S].x, av[S].y - sy[S].y - delta * y[S].y}
                            : cplx<T>{T(0), T(0)};
         });
         fwd_upk(r);
@@ -759,9 +770,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           constexpr int S = decltype(S_)::value;
           y[S].x += dy[S].x;
           y[S].y += dy[S].y;
-          if (cs[S] < L) ay += y[S].x;
+          if (cs[S] < L) ay += av[S].x * y[S].x + av[S].y * y[S].y;   // Re(a^H y)
         });
-        accP = Gp::sum(ay);   // a^H y (real part) of the refined solution
+        accP = Gp::sum(ay);   // Re(a^H y) of the refined solution
         if (active && g == 0 && a.P && (!OPTS || a.pmode == 0))
           a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
       }

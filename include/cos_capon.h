@@ -101,8 +101,8 @@ typedef struct {
     int z_mode;          /* COS_Z_*: amplitude output */
     int p_mode;          /* COS_P_*: power output */
     int refine;          /* nonzero: one step of iterative refinement on the FP32 paths, the
-                            residual a - R_L y taken from the snapshots themselves (no
-                            steering, no forward-backward averaging) */
+                            residual a - R_L y taken from the snapshots themselves (a = 1 or
+                            the steering vector; no forward-backward averaging) */
     int cov_mfma;        /* nonzero: covariance on the matrix cores (BF16 hi/lo split, FP32
                             accumulation) into R (or a plan buffer when R is NULL), then the
                             FP32 solve from it; uniform L <= 32, default options */
