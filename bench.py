@@ -98,8 +98,10 @@ def cpu_baseline(x, budget_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # a frame takes ~1.7 ms: 50 untimed frames bring the clocks to steady state before the
+    # 100 timed ones (0.25 s of GPU time in all)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparator")
     ap.add_argument("--workload", default="frame",
