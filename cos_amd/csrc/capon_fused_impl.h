@@ -487,14 +487,18 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     });
     // trailing update A[i][c] -= conj(U[k][i]) U[k][c]
     if constexpr (DPP_FMA) {
-      // U[k][i] broadcast inside the FMAs from its owner lane
+      // U[k][i] broadcast inside the FMAs from its owner lane.  The broadcast source, row k
+      // of slot i/G, is written once at the top of the step; only the first update that
+      // reads each slot needs the DPP read's wait states, the later ones follow at least one
+      // block of FMAs that does not write it.
       static_for<k + 1, LMAX>([&](auto Ii_) {
         constexpr int i = decltype(Ii_)::value, si = i / G, li = i % G;
+        constexpr bool first = i == k + 1 || i == G;
         if constexpr (CPL == 2 && i < G)
-          sub_conj_bcast_mul2<li, true>(A[slot<G, 0, i>()], A[slot<G, 1, i>()], A[slot<G, si, k>()],
-                                        A[slot<G, 0, k>()], A[slot<G, 1, k>()]);
+          sub_conj_bcast_mul2<li, first>(A[slot<G, 0, i>()], A[slot<G, 1, i>()], A[slot<G, si, k>()],
+                                         A[slot<G, 0, k>()], A[slot<G, 1, k>()]);
         else
-          sub_conj_bcast_mul<li, true>(A[slot<G, CPL - 1, i>()], A[slot<G, si, k>()], A[slot<G, CPL - 1, k>()]);
+          sub_conj_bcast_mul<li, first>(A[slot<G, CPL - 1, i>()], A[slot<G, si, k>()], A[slot<G, CPL - 1, k>()]);
       });
     } else {
       // the first rows take U[k][i] by a register broadcast, which covers the latency of
