@@ -729,18 +729,20 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
                 xv[l] = ldx<T>(xt + min(kb + g + l, M - 1));
               });
               cplx<T> nb = {T(0), T(0)};   // -(y^H u_{t,k}) = -conj(beta_{t,k})
+              // the broadcast sources (y, then nb) are written before each loop: only their
+              // first reads need the DPP wait states
               static_for<0, LMAX>([&](auto l_) {
                 constexpr int l = decltype(l_)::value;
-                bcast_sub_conj_mul<G, l % G>(nb, y[l / G], xv[l]);
+                bcast_sub_conj_mul<G, l % G, l % G == 0>(nb, y[l / G], xv[l]);
               });
               if (kb + g >= K) nb = {T(0), T(0)};
               // (S y)_c += x_t[k + c] beta_{t,k} for the G subarrays of the block
               static_for<0, G>([&](auto l2_) {
                 constexpr int l2 = decltype(l2_)::value;
                 if constexpr (CPL == 2)
-                  bcast_sub_conj_mul2<G, l2>(sy[0], sy[1], nb, xv[l2], xv[l2 + G]);
+                  bcast_sub_conj_mul2<G, l2, l2 == 0>(sy[0], sy[1], nb, xv[l2], xv[l2 + G]);
                 else
-                  bcast_sub_conj_mul<G, l2>(sy[0], nb, xv[l2]);
+                  bcast_sub_conj_mul<G, l2, l2 == 0>(sy[0], nb, xv[l2]);
               });
             }
           }
@@ -885,11 +887,13 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
               constexpr int l = decltype(l_)::value;
               static_for<0, (NA + 1) / 2>([&](auto P_) {
                 constexpr int S = 2 * decltype(P_)::value;
+                // y is written before the loop: only the first read of each slot waits
+                constexpr bool nop = l % G == 0 && S == 0;
                 if constexpr (S + 1 < NA)
-                  bcast_sub_conj_mul2<G, l % G>(nb[S], nb[S + 1], y[l / G], ldx<T>(xt + g + G * S + l),
-                                            ldx<T>(xt + g + G * (S + 1) + l));
+                  bcast_sub_conj_mul2<G, l % G, nop>(nb[S], nb[S + 1], y[l / G], ldx<T>(xt + g + G * S + l),
+                                                     ldx<T>(xt + g + G * (S + 1) + l));
                 else
-                  bcast_sub_conj_mul<G, l % G>(nb[S], y[l / G], ldx<T>(xt + g + G * S + l));
+                  bcast_sub_conj_mul<G, l % G, nop>(nb[S], y[l / G], ldx<T>(xt + g + G * S + l));
               });
             });
             static_for<0, KS>([&](auto S_) {
@@ -914,11 +918,13 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
             constexpr int S0 = l < G ? 0 : (l - G + 1 + G - 1) / G;   // first slot with l <= G*S + G - 1
             static_for<S0, KS, 2>([&](auto S_) {
               constexpr int S = decltype(S_)::value;
+              // yc is written before the loop over t: only the first read of each slot waits
+              constexpr bool nop = l % G == 0 && S == S0;
               if constexpr (S + 1 < KS)
-                bcast_sub_conj_mul2<G, l % G>(ng[S], ng[S + 1], yc[l / G], beta[g + G * S - l],
-                                          beta[g + G * (S + 1) - l]);
+                bcast_sub_conj_mul2<G, l % G, nop>(ng[S], ng[S + 1], yc[l / G], beta[g + G * S - l],
+                                                   beta[g + G * (S + 1) - l]);
               else
-                bcast_sub_conj_mul<G, l % G>(ng[S], yc[l / G], beta[g + G * S - l]);
+                bcast_sub_conj_mul<G, l % G, nop>(ng[S], yc[l / G], beta[g + G * S - l]);
             });
           });
           cplx<T> gacc[KS];

@@ -189,23 +189,25 @@ __device__ __forceinline__ void sub_conj_bcast_mul2(cplx<float>& a, cplx<float>&
 #undef COS_FMAC_BC
 
 // a -= conj(u_L) v (and b -= conj(u_L) w), u_L = u in lane L of the lane's G-lane group:
-// the fused DPP form (with its wait states) for FP32 16-lane groups, an explicit
-// broadcast otherwise.  Nothing broadcast stays live in registers between calls.
-template <int G, int L, typename T>
+// the fused DPP form for FP32 16-lane groups, an explicit broadcast otherwise.  Nothing
+// broadcast stays live in registers between calls.  NOP as for sub_conj_bcast_mul: false
+// only where u was not written by either of the two preceding instructions (a later
+// use of a source that an earlier call already waited for).
+template <int G, int L, bool NOP = true, typename T>
 __device__ __forceinline__ void bcast_sub_conj_mul(cplx<T>& a, cplx<T> u, cplx<T> v) {
   if constexpr (G == 16 && std::is_same<T, float>::value) {
-    sub_conj_bcast_mul<L, true>(a, u, v);
+    sub_conj_bcast_mul<L, NOP>(a, u, v);
   } else {
     sub_conj_mul(a, bcastc<G, L>(u), v);
   }
 }
-template <int G, int L, typename T>
+template <int G, int L, bool NOP = true, typename T>
 __device__ __forceinline__ void bcast_sub_conj_mul2(cplx<T>& a, cplx<T>& b, cplx<T> u, cplx<T> v, cplx<T> w) {
   if constexpr (G == 16 && std::is_same<T, float>::value) {
-    sub_conj_bcast_mul2<L, true>(a, b, u, v, w);
+    sub_conj_bcast_mul2<L, NOP>(a, b, u, v, w);
   } else {
-    bcast_sub_conj_mul<G, L>(a, u, v);
-    bcast_sub_conj_mul<G, L>(b, u, w);
+    bcast_sub_conj_mul<G, L, NOP>(a, u, v);
+    bcast_sub_conj_mul<G, L, NOP>(b, u, w);
   }
 }
 
