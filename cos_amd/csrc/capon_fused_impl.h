@@ -509,15 +509,27 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         if (SKIP && i >= Lw) return;
         ur[i] = rb[i];
       });
-      static_for<k + 1, LMAX>([&](auto Ii_) {
+      auto row_update = [&](auto Ii_) {
         constexpr int i = decltype(Ii_)::value;
-        if (SKIP && i >= Lw) return;
         const cplx<T> u = (i <= k + CRR) ? bcastc<G, i % G>(A[slot<G, i / G, k>()]) : ur[i];
         static_for<0, CPL>([&](auto S_) {
           constexpr int S = decltype(S_)::value;
           if constexpr (i < G * (S + 1)) sub_conj_mul(A[slot<G, S, i>()], u, A[slot<G, S, k>()]);
         });
-      });
+      };
+      if constexpr (SKIP) {
+        // the wave steps only to its own L: rows in blocks of four with one uniform test per
+        // block (a test per row cost a scalar compare and branch for every step and row);
+        // rows at or past L in the last block meet U[k][i] = 0 from the identity padding
+        // and change nothing
+        static_for<0, (LMAX - k - 1 + 3) / 4>([&](auto B_) {
+          constexpr int i0 = k + 1 + 4 * decltype(B_)::value;
+          if (i0 >= Lw) return;
+          static_for<i0, (i0 + 4 < LMAX ? i0 + 4 : LMAX)>(row_update);
+        });
+      } else {
+        static_for<k + 1, LMAX>(row_update);
+      }
     }
   });
   // a clamped pivot gives 1/U_kk = rsqrt(tiny): the factorisation broke down
