@@ -13,13 +13,17 @@
 // The delayed channels v_n are also written as the cube [n_x][n_y][n_h], which is the
 // layout the Capon kernels read ([beam][range][element]).
 //
-// One wave per run of TD_PPW consecutive pixels (the origin sums are formed once per
-// wave), lanes over hydrophones (the cube row of a pixel is written with one coalesced
-// store per 64 hydrophones).  Geometry, interpolation index and the phase's reduction to
-// [0, 2 pi) are evaluated in FP64: t ~ 0.1 s and fc ~ 1e5 Hz put the phase near 1e4 rad,
-// where FP32 rounding of t alone would cost ~1e-2 rad.  The interpolation weight, the
-// interpolation, the sine and cosine of the reduced phase and the rotation are FP32
-// (relative error ~3e-7 against the FP64 formula), the image sum of 64 terms too.
+// A block of four waves owns 64 consecutive range pixels of one beam; lanes run over the
+// pixels and the waves over the hydrophones (interleaved), so for one hydrophone the lanes
+// read neighbouring samples of its channel (neighbouring pixels have neighbouring delays:
+// coalesced reads).  The delayed values of a chunk of 64 hydrophones go through an LDS
+// tile [pixel][hydrophone] (rows padded against bank conflicts) and leave as the cube's
+// contiguous [pixel][hydrophone] rows with coalesced stores; the image sums meet in LDS.
+// Geometry, interpolation index and the phase's reduction to [0, 2 pi) are evaluated in
+// FP64: t ~ 0.1 s and fc ~ 1e5 Hz put the phase near 1e4 rad, where FP32 rounding of t
+// alone would cost ~1e-2 rad.  The interpolation weight, the interpolation, the sine and
+// cosine of the reduced phase and the rotation are FP32 (relative error ~3e-7 against the
+// FP64 formula), the image sum too.
 #include "cos_device.h"
 #include "aux_stages.h"
 
@@ -27,13 +31,18 @@ namespace cosk {
 
 namespace {
 
-constexpr int TD_PPW = 8;   // consecutive pixels per wave
+constexpr int TD_TP = 64;          // pixels per block (one per lane)
+constexpr int TD_HC = 64;          // hydrophones per LDS tile
+constexpr int TD_PAD = TD_HC + 1;  // tile row stride in float2
 
-__global__ __launch_bounds__(256) void tdib_kernel(const TdibLaunch a) {
-  const int lane = threadIdx.x % 64;
-  const long npx = (long)a.n_x * a.n_y;
-  const long p0 = ((long)blockIdx.x * 4 + threadIdx.x / 64) * TD_PPW;
-  if (p0 >= npx) return;
+__global__ __launch_bounds__(256) void tdib_kernel(const TdibLaunch a, const int tiles_per_beam) {
+  __shared__ float2 tile[TD_TP * TD_PAD];
+  __shared__ float2 part[4][TD_TP];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nx = blockIdx.x / tiles_per_beam, ny0 = (blockIdx.x - nx * tiles_per_beam) * TD_TP;
+  const int ny = ny0 + lane;
+  const bool valid = ny < a.n_y;
+  const long pix = (long)nx * a.n_y + (valid ? ny : ny0);
   // origin: xx = mean(x_r)/2, zz = mean(D)/2 (a wave sum; n_h is small)
   double sx = 0.0, sd = 0.0;
   for (int n = lane; n < a.n_h; n += 64) {
@@ -42,17 +51,16 @@ __global__ __launch_bounds__(256) void tdib_kernel(const TdibLaunch a) {
   }
   const double xx = Grp<64>::sum(sx) / (2.0 * a.n_h), zz = Grp<64>::sum(sd) / (2.0 * a.n_h);
   const double inv_dt = 1.0 / a.dt;
-  for (int k = 0; k < TD_PPW; ++k) {
-    const long pix = p0 + k;
-    if (pix >= npx) break;
-    const int nx = (int)(pix / a.n_y), ny = (int)(pix - (long)nx * a.n_y);
-    const double xa = a.xarr[pix], ya = a.yarr[pix];
-    const double ex = xx + xa;
-    const double r_t = sqrt(ex * ex + ya * ya + zz * zz);
-    float2 acc = make_float2(0.f, 0.f);
-    for (int n = lane; n < a.n_h; n += 64) {
+  const double xa = a.xarr[pix], ya = a.yarr[pix];
+  const double ex = xx + xa, ya2 = ya * ya;
+  const double r_t = sqrt(ex * ex + ya2 + zz * zz);
+  float2 acc = make_float2(0.f, 0.f);
+  for (int n0 = 0; n0 < a.n_h; n0 += TD_HC) {
+    const int nc = min(TD_HC, a.n_h - n0);
+    for (int j = w; j < nc; j += 4) {
+      const int n = n0 + j;
       const double dx = ex - a.x_r[n], dz = zz - a.D[n];
-      const double r_r = sqrt(dx * dx + ya * ya + dz * dz);
+      const double r_r = sqrt(dx * dx + ya2 + dz * dz);
       const double t = (r_t + r_r) * a.inv_c;
       // linear interpolation on the uniform axis t0 + i dt, i < n_t
       const double u = (t - a.t0) * inv_dt;
@@ -74,12 +82,26 @@ __global__ __launch_bounds__(256) void tdib_kernel(const TdibLaunch a) {
       }
       acc.x += v.x;
       acc.y += v.y;
-      if (a.cube) a.cube[((size_t)nx * a.n_y + ny) * a.n_h + n] = v;
+      tile[lane * TD_PAD + j] = v;
     }
-    if (a.image) {
-      acc.x = Grp<64>::sum(acc.x);
-      acc.y = Grp<64>::sum(acc.y);
-      if (lane == 0) a.image[(size_t)ny * a.n_x + nx] = make_float2(acc.x / a.n_h, acc.y / a.n_h);
+    __syncthreads();
+    if (a.cube) {
+      // the tile's rows are contiguous in the cube: [pixel][hydrophone n0 .. n0+nc-1]
+      float2* dst = a.cube + ((size_t)nx * a.n_y + ny0) * a.n_h + n0;
+      for (int e = threadIdx.x; e < TD_TP * nc; e += 256) {
+        const int pp = e / nc, jj = e - pp * nc;
+        if (ny0 + pp < a.n_y) dst[(size_t)pp * a.n_h + jj] = tile[pp * TD_PAD + jj];
+      }
+    }
+    __syncthreads();
+  }
+  if (a.image) {
+    part[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && valid) {
+      const float2 s1 = part[1][lane], s2 = part[2][lane], s3 = part[3][lane];
+      const float re = (acc.x + s1.x) + (s2.x + s3.x), im = (acc.y + s1.y) + (s2.y + s3.y);
+      a.image[(size_t)ny * a.n_x + nx] = make_float2(re / a.n_h, im / a.n_h);
     }
   }
 }
@@ -87,11 +109,11 @@ __global__ __launch_bounds__(256) void tdib_kernel(const TdibLaunch a) {
 }  // namespace
 
 hipError_t launch_tdib(const TdibLaunch& a, hipStream_t s) {
-  const long npx = (long)a.n_x * a.n_y;
-  if (npx <= 0) return hipSuccess;
-  const long blocks = (npx + 4 * TD_PPW - 1) / (4 * TD_PPW);
+  if ((long)a.n_x * a.n_y <= 0 || a.n_h <= 0) return hipSuccess;
+  const int tiles_per_beam = (a.n_y + TD_TP - 1) / TD_TP;
+  const long blocks = (long)a.n_x * tiles_per_beam;
   if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(tdib_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(tdib_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, tiles_per_beam);
   return hipGetLastError();
 }
 

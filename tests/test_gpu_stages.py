@@ -26,12 +26,15 @@ def _scene(seed, n_h=32, n_x=8, n_y=300, n_t=6000):
     return mf, mtaxe, x_r, D, xarr, yarr
 
 
-def test_tdib_cube_and_image_match_oracle():
+@pytest.mark.parametrize("n_h,n_x,n_y", [(32, 8, 300), (80, 3, 128), (1, 2, 70)])
+def test_tdib_cube_and_image_match_oracle(n_h, n_x, n_y):
     """Cube and image against the FP64 oracle: within 2e-6 of the data scale (float32 data,
-    FP64 geometry and phase, float32 outputs); NaN exactly where the time leaves the axis."""
+    FP64 geometry and phase, float32 outputs); NaN exactly where the time leaves the axis.
+    Shapes: a partial last pixel tile, two hydrophone chunks (the second partial), one
+    hydrophone."""
     from cos_amd import stages
-    say("tdib", 32, 8, 300)
-    mf, mtaxe, x_r, D, xarr, yarr = _scene(1)
+    say("tdib", n_h, n_x, n_y)
+    mf, mtaxe, x_r, D, xarr, yarr = _scene(1, n_h=n_h, n_x=n_x, n_y=n_y)
     c, fc = 1500.0, 1.0e5
     cube, image = stages.tdib_cube(mf, mtaxe, x_r, D, c, fc, xarr, yarr)
     cube = cube.cpu().numpy()
