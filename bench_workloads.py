@@ -57,8 +57,8 @@ def aperture(args):
     npx = Nx * (Nz - 2 * Y)
     out = {"P": torch.empty(npx, device="cuda"), "z": torch.empty(npx, dtype=torch.complex64, device="cuda"),
            "w": torch.empty((npx, 64), dtype=torch.complex64, device="cuda")}
-    step = lambda refine=False: plan.run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", out=out,
-                                         refine=refine)
+    step = lambda refine=False, mf=False: plan.run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", out=out,
+                                                   refine=refine, cov_mfma=mf)
     for _ in range(args.warmup):
         step()
     ms_default = _timed(torch, step, args.steps)
@@ -89,6 +89,10 @@ def aperture(args):
     # one step of iterative refinement on the FP32 runs (FP64 fallback as before)
     ms_ref = _timed(torch, lambda: step(True), max(3, args.steps // 4))
     er = (out["P"].double() - o64["P"].double()).abs() / o64["P"].double()
+    # the covariance on the matrix cores (BF16 hi/lo split) per row run, then the FP32 solves
+    # from it (FP64 fallback as before)
+    ms_mf = _timed(torch, lambda: step(mf=True), max(3, args.steps // 4))
+    em = (out["P"].double() - o64["P"].double()).abs() / o64["P"].double()
     line = _base(args, "Capon pixels/sec, expanding aperture M(z)=16..128, L(z)=M(z)/2, FP32 + FP64 fallback",
                  npx / (ms * 1e-3), "pixels/s", ms,
                  {"workload": "expanding-aperture frame", "Nx": Nx, "Nz": Nz, "Mmax": Mmax, "Yavg": Y,
@@ -110,6 +114,9 @@ def aperture(args):
                         "changed": {l: s for l, s in tuned["shapes"].items() if s != default[l]}}
     line["refined"] = {"ms_per_frame": round(ms_ref, 3), "vs_all_fp64_P_rel_err_max": float(er.max()),
                        "vs_all_fp64_P_rel_err_mean": float(er.mean())}
+    line["mfma_covariance"] = {"ms_per_frame": round(ms_mf, 3), "vs_all_fp64_P_rel_err_max": float(em.max()),
+                               "vs_all_fp64_P_rel_err_mean": float(em.mean()),
+                               "path": "BF16 hi/lo MFMA covariance per row run -> R in HBM -> FP32 solves"}
     return line
 
 

@@ -155,7 +155,7 @@ class CaponPlan:
         p_mode: "loaded" (1/(a^H R_L^-1 a)) or "unloaded" (|w^H R_hat w|).  refine: one step
         of iterative refinement on the FP32 paths (residual from the snapshots).  cov_mfma:
         covariance on the matrix cores (BF16 hi/lo split), then the FP32 solve from it
-        (uniform L <= 32, default options).  Returns a dict of output tensors.
+        (L <= 64, row tables allowed, default options).  Returns a dict of output tensors.
         """
         ds, o, ptr, wstride, stream = self._prepare(
             x, d, L, Yavg, M=M, off=off, precision=precision, want=want, rows=rows, steer_sin=steer_sin,

@@ -255,9 +255,9 @@ int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
     return fail(hipErrorInvalidValue, "steering takes the default z and P definitions");
   if (ds.refine && (ds.precision == COS_PREC_FP64 || ds.fb_avg))
     return fail(hipErrorInvalidValue, "refinement applies to the FP32 paths without forward-backward averaging");
-  if (ds.cov_mfma && (tables || ds.L > 32 || ds.precision == COS_PREC_FP64 || ds.nsteer > 0 || ds.fb_avg ||
-                      ds.z_mode || ds.p_mode || ds.refine))
-    return fail(hipErrorInvalidValue, "the MFMA covariance takes a uniform L <= 32, an FP32 path and default options");
+  if (ds.cov_mfma && (ds.precision == COS_PREC_FP64 || ds.nsteer > 0 || ds.fb_avg || ds.z_mode || ds.p_mode ||
+                      ds.refine))
+    return fail(hipErrorInvalidValue, "the MFMA covariance takes an FP32 path and the default options");
   return 0;
 }
 
@@ -290,13 +290,26 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
   const char* why = nullptr;
   hipError_t e;
   if (ds.cov_mfma) {
-    // covariance on the matrix cores into R (the caller's, or the plan's), then the solve
-    // reads it back
+    // covariance on the matrix cores into R (the caller's, or the plan's; Lmax^2 per pixel),
+    // then the solve reads it back; one launch per row run, with one or four 32 x 32 tiles
     cosk::CaponLaunch c = a;
     c.R = R ? (float2*)R : plan->R.as<float2>();
-    if ((e = cosk::launch_cov_mfma(c, s)) != hipSuccess) return fail_hip(e, "MFMA covariance");
+    const int Rs = Lmax * Lmax;
+    if (tables && !plan->runs.empty()) {
+      for (const RowRun& r : plan->runs) {
+        c.jlo = r.jlo;
+        c.jn = r.jn;
+        c.npix = ds.Nx * r.jn;
+        c.Lcap = r.Lmax;   // every row of the run holds its L within the run's shape
+        if ((e = cosk::launch_cov_mfma(c, r.Lmax > 32 ? 2 : 1, Mmax, Rs, s)) != hipSuccess)
+          return fail_hip(e, "MFMA covariance");
+      }
+    } else if ((e = cosk::launch_cov_mfma(c, Lmax > 32 ? 2 : 1, Mmax, Rs, s)) != hipSuccess) {
+      return fail_hip(e, "MFMA covariance");
+    }
     a.Rin = c.R;
     a.R = nullptr;
+    a.Rstride = Rs;
   }
   if (ds.precision == COS_PREC_FP64) {
     if (tables && !plan->runs.empty()) {
@@ -361,8 +374,12 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   const bool fb = fallback_needed(ds);
   const int npix = ds.Nx * (ds.Nz - 2 * ds.Yavg);
   hipError_t e;
-  if (ds.cov_mfma && !R && (e = plan->R.ensure((size_t)npix * ds.L * ds.L * sizeof(cos_cf32))) != hipSuccess)
-    return fail_hip(e, "covariance buffer");
+  if (ds.cov_mfma && !R) {
+    const bool tab = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+    const size_t Lm = tab ? ds.Lmax : ds.L;
+    if ((e = plan->R.ensure((size_t)npix * Lm * Lm * sizeof(cos_cf32))) != hipSuccess)
+      return fail_hip(e, "covariance buffer");
+  }
   if (fb) {
     if (!flags && (e = plan->flags.ensure(npix)) != hipSuccess) return fail_hip(e, "flags buffer");
     if ((e = plan->list.ensure(sizeof(int) * (size_t)npix)) != hipSuccess) return fail_hip(e, "list buffer");

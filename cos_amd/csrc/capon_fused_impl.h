@@ -173,7 +173,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   bool from_r = false;
   if constexpr (FR) from_r = a.Rin != nullptr;
   if (from_r) {
-    const float2* Rp = a.Rin + (size_t)p * L * L;
+    const float2* Rp = a.Rin + (size_t)p * a.Rstride;
     static_for<0, CPL>([&](auto S_) {
       constexpr int S = decltype(S_)::value;
       static_for<0, G * (S + 1)>([&](auto R_) {

@@ -43,14 +43,15 @@ struct CaponLaunch {
   // options: forward-backward averaging, power-Capon amplitude, unloaded power
   int fb, zmode, pmode;
   // optional loaded covariance per pixel in the reference R layout (mem[r*L+c] =
-  // conj(S_L[r][c]), stride L*L, uniform L): the runtime-window FP32 kernels then solve
-  // with it instead of building the covariance
+  // conj(S_L[r][c]) for the pixel's L, pixel stride Rstride): the runtime-window FP32
+  // kernels then solve with it instead of building the covariance
   const float2* Rin;
 };
 
-// Loaded covariance of every pixel on the matrix cores (BF16 hi/lo split, FP32
-// accumulation) into a.R in the reference layout; uniform L <= 32.
-hipError_t launch_cov_mfma(const CaponLaunch& a, hipStream_t s);
+// Loaded covariance of the pixels of output rows jlo .. jlo+jn-1 (all beams) on the matrix
+// cores (BF16 hi/lo split, FP32 accumulation) into a.R + p*Rs in the reference layout;
+// ntl 32 x 32 tiles per side (L <= 32 ntl), per-row tables as in CaponLaunch.
+hipError_t launch_cov_mfma(const CaponLaunch& a, int ntl, int Mmax, int Rs, hipStream_t s);
 
 // Lane shapes of the FP32 fused kernel, the autotuner's choices per subarray length:
 // 1 = 16 lanes x 1 column (L <= 16), 2 = 16 lanes x 2 columns (L <= 32), 3 = 64 lanes

@@ -105,7 +105,8 @@ typedef struct {
                             the steering vector; no forward-backward averaging) */
     int cov_mfma;        /* nonzero: covariance on the matrix cores (BF16 hi/lo split, FP32
                             accumulation) into R (or a plan buffer when R is NULL), then the
-                            FP32 solve from it; uniform L <= 32, default options */
+                            FP32 solve from it; L <= 64, row tables allowed (R then a plan
+                            buffer of Lmax*Lmax per pixel), default options */
 } cos_capon_desc;
 
 enum {

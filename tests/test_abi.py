@@ -76,7 +76,7 @@ def test_plan_run_rejects_bad_descriptor():
         code, text = _err(L.cos_capon_run(h, C.byref(ds), 16, None, None, None, 0, None, None, None))
         assert code != 0 and "refinement applies to the FP32 paths" in text
     # the matrix-core covariance covers uniform L <= 32 with the default options
-    for kw in (dict(L=40), dict(refine=1), dict(p_mode=1)):
+    for kw in (dict(L=65, M=70, Mstride=70), dict(refine=1), dict(p_mode=1)):
         args = dict(Nx=1, Nz=8, Mstride=64, M=64, L=16, off=0, Yavg=1, d=0.1, cov_mfma=1)
         args.update(kw)
         ds = CaponDesc(**args)

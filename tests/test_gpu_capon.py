@@ -205,7 +205,8 @@ def test_refined_rows_tables_with_fallback():
 
 # ------------------------------------------------------------------ matrix-core covariance
 
-@pytest.mark.parametrize("M,L,Y", [(64, 32, 2), (24, 12, 1), (40, 17, 0), (16, 1, 2)])
+@pytest.mark.parametrize("M,L,Y", [(64, 32, 2), (24, 12, 1), (40, 17, 0), (16, 1, 2), (80, 40, 1), (128, 64, 2),
+                                   (70, 33, 2)])
 def test_mfma_covariance_matches_oracle(M, L, Y):
     """The covariance on the matrix cores (BF16 hi/lo split, FP32 accumulation) in the
     reference R layout (loaded, unnormalised, conj-stored) against the FP64 oracle: within
@@ -233,6 +234,28 @@ def test_mfma_covariance_cfg2_slice(cfg2_slice):
     e = rel(o["P"], ref["P"].ravel())
     print(f"mfma-covariance P err max {e.max():.3e} mean {e.mean():.3e}", flush=True)
     assert e.max() <= 1e-4
+
+
+def test_mfma_covariance_aperture_rows():
+    """The expanding-aperture geometry (L 8..64 by row, 5% of rows at d = 1e-5) through the
+    matrix-core covariance, one launch per row run with one or four tiles, then the FP32
+    solves and the FP64 pass over the flagged pixels: power within 1e-4 of FP64."""
+    import torch
+    from cos_amd import synth
+    say("mfma covariance aperture", 2, 1024, 128)
+    Nx, Nz, Y = 2, 1024, 2
+    Mz, Lz, off, dz = synth.config3_geometry(Nz, seed=3, Mmax=128)
+    x = synth.speckle_cube(Nx, Nz, 128, seed=3)
+    rows = {k: torch.from_numpy(v).cuda() for k, v in dict(L=Lz, M=Mz, off=off, d=dz).items()}
+    o, plan = _plan_run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", cov_mfma=True, want=("P", "w", "flags"))
+    Nzp = Nz - 2 * Y
+    sel = np.arange(0, Nx * Nzp, 5)
+    ref = O.capon_rows(x.astype(np.complex128), dz.astype(np.float64), Lz, Mz, off, Y,
+                       pixels=(sel // Nzp, sel % Nzp))
+    e = rel(o["P"][sel], ref["P"])
+    print(f"mfma aperture P rel err max {e.max():.3e}; fallback {plan.fallback_count()}", flush=True)
+    assert e.max() <= 1e-4
+    assert plan.fallback_count() > 0
 
 
 # ------------------------------------------------------------------ lane-shape autotuner
