@@ -9,8 +9,13 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s);
 template <int G, int CPL>
 static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
   const int TW = 2 * a.Y + 1;
-  // one step of iterative refinement: its own runtime-window kernels (more LDS)
-  if (a.refine) return launch_one<float, G, CPL, 0, false, false, 0, true>(a, Mmax, s);
+  // one step of iterative refinement: its own kernels, with the fixed window bound when
+  // no reference option needs the runtime-window kernels (nor a stored covariance)
+  if (a.refine) {
+    if (TW <= 5 && !(a.fb || a.zmode || a.pmode || a.Rin))
+      return launch_one<float, G, CPL, 5, false, false, 0, true>(a, Mmax, s);
+    return launch_one<float, G, CPL, 0, false, false, 0, true>(a, Mmax, s);
+  }
   // the optional variants live in the runtime-window kernels
   if (a.fb || a.zmode || a.pmode || a.Rin) return launch_one<float, G, CPL, 0, false, false, 0>(a, Mmax, s);
   if (TW <= 5) return launch_one<float, G, CPL, 5, false, false, false>(a, Mmax, s);
