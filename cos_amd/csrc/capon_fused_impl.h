@@ -876,6 +876,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         __syncthreads();
         // lane slots k = g + G*S that hold some k < K: KA = ceil(K/G), uniform
         const int KA = (K + G - 1) / G;
+        // a short last block of subarrays (K = 33 at L = 32 leaves one) is done by group
+        // sums instead of a whole lane-parallel slot
+        const int KF = K / G, KR = K - KF * G;
+        const bool short_tail = KF >= 1 && KR > 0 && KR < G / 4;
+        const int KAe = short_tail ? KF : KA;
         cplx<T> yc[CPL];
         static_for<0, CPL>([&](auto S_) {
           constexpr int S = decltype(S_)::value;
@@ -889,7 +894,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           // elements past the last row, inside the group's region), and are not kept
           static_for<1, KS + 1>([&](auto KA_) {
             constexpr int NA = decltype(KA_)::value;
-            if (KA != NA) return;
+            if (KAe != NA) return;
             cplx<T> nb[NA];   // -beta
             static_for<0, NA>([&](auto S_) {
               constexpr int S = decltype(S_)::value;
@@ -916,6 +921,20 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
               beta[k] = k < K ? v : cplx<T>{T(0), T(0)};
             });
           });
+          if (short_tail) {
+            for (int k = KF * G; k < K; ++k) {   // beta_k = sum_c conj(y_c) x_t[k + c]
+              T br = T(0), bi = T(0);
+              static_for<0, CPL>([&](auto S_) {
+                constexpr int S = decltype(S_)::value;
+                const cplx<T> xv = ldx<T>(xt + k + cs[S]);
+                br = fma(y[S].x, xv.x, fma(y[S].y, xv.y, br));
+                bi = fma(y[S].x, xv.y, fma(-y[S].y, xv.x, bi));
+              });
+              br = Gp::sum(br);
+              bi = Gp::sum(bi);
+              if (g == k - KF * G) beta[k] = {br, bi};
+            }
+          }
           __syncthreads();
           // gacc_m = sum_l y_l beta_{t,m-l}, accumulated negated as -conj(conj(y_l)) beta;
           // slot S holds m < G(S+1), so l > G(S+1)-1 meets only the zero prefix and is
