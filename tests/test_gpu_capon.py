@@ -152,7 +152,8 @@ def test_refinement_on_ill_conditioned_covariance():
     assert np.median(we(o1["w"])) <= np.median(we(o0["w"]))
 
 
-@pytest.mark.parametrize("M,L,Y", [(24, 12, 1), (48, 24, 2), (80, 40, 1), (128, 64, 2), (16, 1, 2)])
+@pytest.mark.parametrize("M,L,Y", [(24, 12, 1), (48, 24, 2), (80, 40, 1), (128, 64, 2), (16, 1, 2), (32, 32, 1),
+                                   (20, 20, 0)])
 def test_refined_shapes_match_oracle(M, L, Y):
     """Every lane shape of the refined kernel (16 lanes x 1 or 2 columns, 64 lanes) against
     the FP64 oracle: P within 1e-4, w within 1e-3 of its largest entry, z within 1e-4."""
@@ -282,6 +283,27 @@ def test_every_lane_shape_matches_oracle(M, L):
                                        atol=1e-3 * np.abs(ref["w"]).max())
     finally:
         CaponPlan.set_tuning(xt, 0.05, L, 2, {})
+
+
+def test_autotune_uniform_frame():
+    """The autotuner on a frame of one subarray length: both shapes that hold L = 24 are
+    timed, the faster is kept, and the tuned frame matches FP64."""
+    import torch
+    from cos_amd import CaponPlan
+    say("autotune uniform", 48, 24)
+    x = uniform_cube(91, 4, 160, 48)
+    xt = torch.from_numpy(x).cuda()
+    plan = CaponPlan()
+    try:
+        tuned = plan.autotune(xt, 0.05, 24, 2)
+        assert list(tuned["shapes"]) == [24] and sorted(tuned["ms"][24]) == [2, 3]
+        assert tuned["ms"][24][tuned["shapes"][24]] == min(tuned["ms"][24].values())
+        o = plan.run(xt, 0.05, 24, 2, precision="fp32+fallback")
+        torch.cuda.synchronize()
+        ref = O.capon(x.astype(np.complex128), 0.05, 24, 2)
+        assert rel(o["P"].cpu().numpy(), ref["P"].ravel()).max() <= 1e-4
+    finally:
+        CaponPlan.set_tuning(xt, 0.05, 24, 2, {})
 
 
 def test_autotune_aperture_rows():
