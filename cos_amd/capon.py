@@ -25,6 +25,8 @@ import numpy as np
 from . import _lib
 from ._lib import CaponDesc, check, lib
 
+N_SHAPES = 4   # lane shapes of the fused FP32 kernel (COS_CAPON_SHAPES in include/cos_capon.h)
+
 
 def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
@@ -237,20 +239,22 @@ class CaponPlan:
         frame configuration and install the fastest for the device (later run() calls with
         the same Mmax, Yavg and options use it).  Arguments as run().  Returns
         {"shapes": {L: shape}, "ms": {L: {shape: ms per launch}}}; shape 1 = 16 lanes x 1
-        column, 2 = 16 lanes x 2 columns, 3 = 64 lanes."""
+        column, 2 = 16 lanes x 2 columns, 3 = 64 lanes, 4 = 16 lanes x 3 columns (L <= 48,
+        not with refine)."""
         ds, o, ptr, wstride, stream = self._prepare(
             x, d, L, Yavg, M=M, off=off, precision=precision, want=("P", "z", "w"), rows=rows, steer_sin=None,
             cond_thresh=1e6, out=None, stream=stream, graph=False, fb_avg=fb_avg, z_mode=z_mode,
             p_mode=p_mode, refine=refine)
         shapes = (C.c_int * 65)()
-        ms = (C.c_float * (65 * 3))()
+        ms = (C.c_float * (65 * N_SHAPES))()
         check(lib().cos_capon_autotune(self._h, C.byref(ds), x.data_ptr(), ptr("P"), ptr("z"), ptr("w"),
                                        wstride, stream, shapes, ms))
         res = {"shapes": {}, "ms": {}}
         for l in range(1, 65):
             if shapes[l]:
                 res["shapes"][l] = int(shapes[l])
-                res["ms"][l] = {s + 1: float(ms[3 * l + s]) for s in range(3) if ms[3 * l + s] >= 0}
+                res["ms"][l] = {s + 1: float(ms[N_SHAPES * l + s]) for s in range(N_SHAPES)
+                                if ms[N_SHAPES * l + s] >= 0}
         return res
 
     @staticmethod

@@ -54,6 +54,16 @@ __device__ __forceinline__ cplx<T> ldx(const float2* p) {
 // other shapes without the DPP-FMA broadcast, which covers the LDS latency of the rest.
 template <int G> struct CholRegRows { static constexpr int value = G == 64 ? 64 : 10; };
 
+// Row buffers of LMAX entries reserved after the packed U: two where row k of U is
+// published in LDS for the trailing update (groups without the DPP broadcast, past the
+// first CholRegRows rows) or where the 64-lane refinement passes y and beta through LDS,
+// one for the w of the runtime-window kernels' options, none otherwise.
+template <typename T, int G, int CPL, int TM, int BWD, bool REF> struct RowBufs {
+  static constexpr bool dpp = std::is_same<T, float>::value && G == 16;
+  static constexpr bool publish = !dpp && CholRegRows<G>::value < G * CPL - 1;
+  static constexpr int value = (publish || (REF && G == 64)) ? 2 : (TM == 0 && !BWD) ? 1 : 0;
+};
+
 template <int G, int S, int R>
 __device__ __forceinline__ constexpr int slot() { return G * S * (S + 1) / 2 + R; }
 
@@ -86,9 +96,10 @@ __device__ __noinline__ cplx<T> steered_mean(const float2* xc, int c, int K, int
 
 // Occupancy target (waves per SIMD) the register allocator is held to.
 // (16-lane groups with one column per lane: 4, or 3 when a window of more than five
-// rows keeps its alpha/beta columns in registers)
+// rows keeps its alpha/beta columns in registers; three columns per lane hold 96 complex
+// entries of the triangle and take the whole register file)
 template <typename T, int G, int CPL, int TM> struct OccHint {
-  static constexpr int value = sizeof(T) == 4 ? (G == 16 ? (CPL == 1 ? (TM == 0 || TM > 5 ? 3 : 4) : 2) : 1)
+  static constexpr int value = sizeof(T) == 4 ? (G == 16 ? (CPL == 1 ? (TM == 0 || TM > 5 ? 3 : 4) : CPL == 2 ? 2 : 1) : 1)
                                               : (G == 16 && CPL == 1 ? 2 : 1);
 };
 
@@ -435,7 +446,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   //   dump NU               target of the row-k writes of lanes whose column is <= k
   //   rowb [NU+2, NU+2+2L)  row k of U, published for the trailing update (broadcast
   //                         reads); two buffers alternate so row k+1 can be published
-  //                         while row k's updates still read the other one
+  //                         while row k's updates still read the other one.  Reserved
+  //                         only where it is used (RowBufs): the FP32 16-lane groups
+  //                         broadcast by DPP and need one buffer for the options' w
+  //                         only, which keeps three columns per lane at four waves per CU
   //   invd (T) at the end   1/U_kk
   // Row k is written to Upk as it is finished, so the registers of A die row by row.
   // FP32 16-lane groups broadcast U[k][i] inside the FMAs (v_fmac_f32 with a DPP source),
@@ -491,14 +505,20 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       // of slot i/G, is written once at the top of the step; only the first update that
       // reads each slot needs the DPP read's wait states, the later ones follow at least one
       // block of FMAs that does not write it.
+      // Row i lives in slots si .. CPL-1; they are updated two at a time sharing the
+      // broadcast, an odd one alone.
       static_for<k + 1, LMAX>([&](auto Ii_) {
         constexpr int i = decltype(Ii_)::value, si = i / G, li = i % G;
-        constexpr bool first = i == k + 1 || i == G;
-        if constexpr (CPL == 2 && i < G)
-          sub_conj_bcast_mul2<li, first>(A[slot<G, 0, i>()], A[slot<G, 1, i>()], A[slot<G, si, k>()],
-                                         A[slot<G, 0, k>()], A[slot<G, 1, k>()]);
-        else
-          sub_conj_bcast_mul<li, first>(A[slot<G, CPL - 1, i>()], A[slot<G, si, k>()], A[slot<G, CPL - 1, k>()]);
+        constexpr bool first = i == k + 1 || i % G == 0;
+        constexpr int ns = CPL - si;
+        if constexpr (ns % 2 == 1)
+          sub_conj_bcast_mul<li, first>(A[slot<G, si, i>()], A[slot<G, si, k>()], A[slot<G, si, k>()]);
+        static_for<0, ns / 2>([&](auto P_) {
+          constexpr int s0 = si + ns % 2 + 2 * decltype(P_)::value;
+          sub_conj_bcast_mul2<li, first && ns % 2 == 0 && s0 == si>(A[slot<G, s0, i>()], A[slot<G, s0 + 1, i>()],
+                                                                    A[slot<G, si, k>()], A[slot<G, s0, k>()],
+                                                                    A[slot<G, s0 + 1, k>()]);
+        });
       });
     } else {
       // the first rows take U[k][i] by a register broadcast, which covers the latency of
@@ -1039,8 +1059,9 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   const int MP = ((Mmax + LMAX + 4 + 3) / 4) * 4;   // zero padding covers every lag read
   const int TW = 2 * a.Y + 1;
   // per-group LDS region: the window's snapshots, later the packed upper U + dump + pad,
-  // two row buffers and 1/U_kk (layout in capon_group)
-  const int upk = (LMAX * (LMAX + 1) / 2 + 2 + 2 * LMAX) * (int)(sizeof(cplx<T>) / sizeof(float2)) +
+  // the row buffers the variant uses and 1/U_kk (layout in capon_group)
+  constexpr int NRB = RowBufs<T, G, CPL, TM, BWD, REF>::value;
+  const int upk = (LMAX * (LMAX + 1) / 2 + 2 + NRB * LMAX) * (int)(sizeof(cplx<T>) / sizeof(float2)) +
                   LMAX * (int)sizeof(T) / (int)sizeof(float2);
   int GREG = TW * MP > upk ? TW * MP : upk;
   if (BWD) {   // window rows plus the zero-padded beta of the adjoint epilogue

@@ -136,18 +136,21 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
  * reference has one kernel configuration).  For each subarray length the frame uses (the
  * uniform L, or each distinct value of the row table) it times every lane shape that holds
  * that length (1: 16 lanes x 1 column, L <= 16; 2: 16 lanes x 2 columns, L <= 32;
- * 3: 64 lanes, L <= 64) on up to 128 output rows of that length (all beams) and keeps the
- * fastest for this device and frame configuration (Mmax, Yavg, options, refine); later
- * cos_capon_run calls on the device use it.  P, z, w (device, nullable, as cos_capon_run)
- * receive the timed rows' results.  shapes_out [65] (nullable): chosen shape per L, 0 where
- * the frame has no row of that length; ms_out [65*3] (nullable): ms per launch of shape s
- * at L in ms_out[3*L + s - 1], negative where not timed.  Synchronous on stream. */
+ * 3: 64 lanes, L <= 64; 4: 16 lanes x 3 columns, L <= 48, not with refine) on up to 128
+ * output rows of that length (all beams) and keeps the fastest for this device and frame
+ * configuration (Mmax, Yavg, options, refine); later cos_capon_run calls on the device use
+ * it.  P, z, w (device, nullable, as cos_capon_run) receive the timed rows' results.
+ * shapes_out [65] (nullable): chosen shape per L, 0 where the frame has no row of that
+ * length; ms_out [65*COS_CAPON_SHAPES] (nullable): ms per launch of shape s at L in
+ * ms_out[COS_CAPON_SHAPES*L + s - 1], negative where not timed.  Synchronous on stream. */
+#define COS_CAPON_SHAPES 4
 int cos_capon_autotune(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x, float* P,
                        cos_cf32* z, cos_cf32* w, int wstride, void* stream, int* shapes_out, float* ms_out);
 
 /* Installs a lane-shape table shapes[65] (0 = the smallest shape that holds L) for the
  * current device and the frame configuration of desc, e.g. one saved from
- * cos_capon_autotune. */
+ * cos_capon_autotune.  Rejects a shape that does not hold its L, and shape 4 when
+ * desc->refine is set. */
 int cos_capon_tuning_set(const cos_capon_desc* desc, const int* shapes);
 
 /* Beamspace Capon with the sliding DFT: replaces the reference's SLIDING_BEAMSPACE build

@@ -261,33 +261,73 @@ def test_mfma_covariance_aperture_rows():
 
 # ------------------------------------------------------------------ lane-shape autotuner
 
-@pytest.mark.parametrize("M,L", [(24, 12), (64, 32)])
-def test_every_lane_shape_matches_oracle(M, L):
+SHAPE_CAP = (16, 32, 64, 48)   # lane shapes 1..4: the largest L each holds
+
+
+@pytest.mark.parametrize("M,L,Y", [(24, 12, 2), (64, 32, 2), (96, 40, 2), (96, 48, 2), (80, 36, 4), (56, 33, 0)])
+def test_every_lane_shape_matches_oracle(M, L, Y):
     """Each lane shape that holds L, forced through the tuning table, against the FP64
-    oracle; the table is reset afterwards."""
+    oracle (windows of 1, 5 and 9 rows: the three fixed-window kernels); the table is
+    reset afterwards."""
     import torch
     from cos_amd import CaponPlan
     x = uniform_cube(5 * M + L, 3, 20, M)
     xt = torch.from_numpy(x).cuda()
-    ref = O.capon(x.astype(np.complex128), 0.05, L, 2)
+    ref = O.capon(x.astype(np.complex128), 0.05, L, Y)
     try:
-        for sh in (1, 2, 3):
-            if (16, 32, 64)[sh - 1] < L:
+        for sh in (1, 2, 3, 4):
+            if SHAPE_CAP[sh - 1] < L:
                 continue
-            say("forced shape", M, L, sh)
-            CaponPlan.set_tuning(xt, 0.05, L, 2, {L: sh})
-            o = CaponPlan().run(xt, 0.05, L, 2, precision="fp32+fallback")
+            say("forced shape", M, L, Y, sh)
+            CaponPlan.set_tuning(xt, 0.05, L, Y, {L: sh})
+            o = CaponPlan().run(xt, 0.05, L, Y, precision="fp32+fallback")
             torch.cuda.synchronize()
             assert rel(o["P"].cpu().numpy(), ref["P"].ravel()).max() <= 1e-4
             np.testing.assert_allclose(o["w"].cpu().numpy(), ref["w"].reshape(-1, L),
                                        atol=1e-3 * np.abs(ref["w"]).max())
+            np.testing.assert_allclose(o["z"].cpu().numpy(), ref["z"].ravel(), atol=1e-4 * np.abs(ref["z"]).max())
     finally:
-        CaponPlan.set_tuning(xt, 0.05, L, 2, {})
+        CaponPlan.set_tuning(xt, 0.05, L, Y, {})
+
+
+@pytest.mark.parametrize("case", ["options", "steering", "mfma_covariance"])
+def test_three_column_shape_runtime_window_kernel(case):
+    """Lane shape 4 (16 lanes x 3 columns) in its runtime-window kernel: the reference
+    options (forward-backward averaging, power amplitude, unloaded power), steering
+    vectors, and the solve from a stored covariance, each against the FP64 oracle."""
+    import torch
+    from cos_amd import CaponPlan, synth
+    M, L, Y = 80, 40, 1
+    say("shape 4", case, M, L, Y)
+    x = uniform_cube(M + 3 * L, 3, 12, M)
+    xt = torch.from_numpy(x).cuda()
+    opts = dict(fb_avg=True, z_mode="power", p_mode="unloaded") if case == "options" else {}
+    try:
+        CaponPlan.set_tuning(xt, 0.05, L, Y, {L: 4}, **opts)
+        if case == "options":
+            o, _ = _plan_run(x, 0.05, L, Y, **opts)
+            ref = O.capon_options(x.astype(np.complex128), 0.05, L, Y, fb=True, z_mode=1, p_mode=1)
+            assert rel(o["P"], ref["P"]).max() <= 1e-4
+            assert np.abs(o["z"] - ref["z"]).max() <= 1e-4 * np.abs(ref["z"]).max()
+            np.testing.assert_allclose(o["w"].reshape(-1, L), ref["w"], atol=1e-3 * np.abs(ref["w"]).max())
+        elif case == "steering":
+            sines = synth.steering_sines(8).astype(np.float32)
+            o, _ = _plan_run(x, 0.05, L, Y, steer_sin=torch.from_numpy(sines).cuda())
+            ref = O.capon_steered(x.astype(np.complex128), 0.05, L, Y, sines.astype(np.float64))
+            assert rel(o["P"].reshape(-1, len(sines)), ref["P"]).max() <= 1e-4
+            w = o["w"].reshape(-1, len(sines), L)
+            assert np.abs(w - ref["w"]).max() <= 1e-3 * np.abs(ref["w"]).max()
+        else:
+            o, _ = _plan_run(x, 0.05, L, Y, cov_mfma=True)
+            ref = O.capon(x.astype(np.complex128), 0.05, L, Y)
+            assert rel(o["P"], ref["P"].ravel()).max() <= 1e-4
+    finally:
+        CaponPlan.set_tuning(xt, 0.05, L, Y, {}, **opts)
 
 
 def test_autotune_uniform_frame():
-    """The autotuner on a frame of one subarray length: both shapes that hold L = 24 are
-    timed, the faster is kept, and the tuned frame matches FP64."""
+    """The autotuner on a frame of one subarray length: the three shapes that hold L = 24
+    are timed, the fastest is kept, and the tuned frame matches FP64."""
     import torch
     from cos_amd import CaponPlan
     say("autotune uniform", 48, 24)
@@ -296,7 +336,7 @@ def test_autotune_uniform_frame():
     plan = CaponPlan()
     try:
         tuned = plan.autotune(xt, 0.05, 24, 2)
-        assert list(tuned["shapes"]) == [24] and sorted(tuned["ms"][24]) == [2, 3]
+        assert list(tuned["shapes"]) == [24] and sorted(tuned["ms"][24]) == [2, 3, 4]
         assert tuned["ms"][24][tuned["shapes"][24]] == min(tuned["ms"][24].values())
         o = plan.run(xt, 0.05, 24, 2, precision="fp32+fallback")
         torch.cuda.synchronize()
@@ -325,7 +365,7 @@ def test_autotune_aperture_rows():
         Ls = sorted(set(Lz[Y:Nz - Y].tolist()))
         assert sorted(tuned["shapes"]) == Ls
         for l, sh in tuned["shapes"].items():
-            valid = [s for s in (1, 2, 3) if (16, 32, 64)[s - 1] >= l]
+            valid = [s for s in (1, 2, 3, 4) if SHAPE_CAP[s - 1] >= l]
             assert sorted(tuned["ms"][l]) == valid
             assert tuned["ms"][l][sh] == min(tuned["ms"][l].values())
         print("tuned shapes", tuned["shapes"], flush=True)
