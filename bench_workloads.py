@@ -108,7 +108,9 @@ def aperture(args):
                         "vs_all_fp64_P_rel_err_mean": float(e.mean()),
                         "vs_oracle_P_rel_err_max_64px": float(eo.max()),
                         "vs_oracle_w_rel_err_max_64px": float(max(ew))}
-    default = {l: (1 if l <= 16 else 2 if l <= 32 else 3) for l in tuned["shapes"]}
+    # the library's untuned rule: the 16-lane shape with the fewest columns that holds L
+    # (1, 2 or 3 columns: shapes 1, 2, 4), else 64 lanes (shape 3)
+    default = {l: (1 if l <= 16 else 2 if l <= 32 else 4 if l <= 48 else 3) for l in tuned["shapes"]}
     line["autotune"] = {"default_shapes_ms_per_frame": round(ms_default, 3), "tuning_s": round(tune_s, 2),
                         "lengths": len(tuned["shapes"]),
                         "changed": {l: s for l, s in tuned["shapes"].items() if s != default[l]}}

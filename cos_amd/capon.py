@@ -239,8 +239,7 @@ class CaponPlan:
         frame configuration and install the fastest for the device (later run() calls with
         the same Mmax, Yavg and options use it).  Arguments as run().  Returns
         {"shapes": {L: shape}, "ms": {L: {shape: ms per launch}}}; shape 1 = 16 lanes x 1
-        column, 2 = 16 lanes x 2 columns, 3 = 64 lanes, 4 = 16 lanes x 3 columns (L <= 48,
-        not with refine)."""
+        column, 2 = 16 lanes x 2 columns, 3 = 64 lanes, 4 = 16 lanes x 3 columns (L <= 48)."""
         ds, o, ptr, wstride, stream = self._prepare(
             x, d, L, Yavg, M=M, off=off, precision=precision, want=("P", "z", "w"), rows=rows, steer_sin=None,
             cond_thresh=1e6, out=None, stream=stream, graph=False, fb_avg=fb_avg, z_mode=z_mode,

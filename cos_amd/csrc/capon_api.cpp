@@ -94,10 +94,10 @@ TuneTable tuned_shapes(const cos_capon_desc& ds) {
   return t;
 }
 
-inline int shape_for(const TuneTable& t, int L, bool refine) {
-  if (L < 1 || L > 64) return cosk::default_shape(L < 1 ? 1 : 64, refine);
+inline int shape_for(const TuneTable& t, int L) {
+  if (L < 1 || L > 64) return cosk::default_shape(L < 1 ? 1 : 64);
   const int s = t[L];
-  return s >= 1 && s <= cosk::kShapes && cosk::shape_fits(s, L, refine) ? s : cosk::default_shape(L, refine);
+  return s >= 1 && s <= cosk::kShapes && cosk::shape_fits(s, L) ? s : cosk::default_shape(L);
 }
 
 // The frame's launches captured once and replayed: valid while the descriptor, every
@@ -344,7 +344,7 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
       if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
     }
   } else {
-    e = cosk::launch_capon_f32_shape(a, shape_for(plan->shapes, Lmax, a.refine != 0), Mmax, s, &why);
+    e = cosk::launch_capon_f32_shape(a, shape_for(plan->shapes, Lmax), Mmax, s, &why);
     if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
   }
   if (!fb) return 0;
@@ -408,9 +408,9 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
       return fail_hip(e, "read L table");
     const int Nzp = ds.Nz - 2 * ds.Yavg;
     for (int jj = 0; jj < Nzp;) {
-      const int sh = shape_for(plan->shapes, hL[jj + ds.Yavg], ds.refine != 0);
+      const int sh = shape_for(plan->shapes, hL[jj + ds.Yavg]);
       int j2 = jj + 1;
-      while (j2 < Nzp && shape_for(plan->shapes, hL[j2 + ds.Yavg], ds.refine != 0) == sh) ++j2;
+      while (j2 < Nzp && shape_for(plan->shapes, hL[j2 + ds.Yavg]) == sh) ++j2;
       const int cap = cosk::shape_cap(sh);
       plan->runs.push_back({jj, j2 - jj, cap < ds.Lmax ? cap : ds.Lmax, sh});
       jj = j2;
@@ -493,7 +493,7 @@ int cos_capon_autotune(cos_capon_plan* plan, const cos_capon_desc* desc, const c
     const int Lr = kv.first;
     float best_ms = 0.f;
     for (int sh = 1; sh <= cosk::kShapes && !rc; ++sh) {
-      if (!cosk::shape_fits(sh, Lr, ds.refine != 0)) continue;
+      if (!cosk::shape_fits(sh, Lr)) continue;
       cosk::CaponLaunch a = base_launch(ds, x);
       a.P = P;
       a.z = (float2*)z;
@@ -545,9 +545,8 @@ int cos_capon_tuning_set(const cos_capon_desc* desc, const int* shapes) {
   TuneTable t{};
   for (int Lr = 1; Lr <= 64; ++Lr) {
     const int sh = shapes[Lr];
-    if (sh < 0 || sh > cosk::kShapes || (sh && !cosk::shape_fits(sh, Lr, desc->refine != 0)))
-      return fail(hipErrorInvalidValue,
-                  "shape table entry does not hold its subarray length (or is shape 4 with refine)");
+    if (sh < 0 || sh > cosk::kShapes || (sh && !cosk::shape_fits(sh, Lr)))
+      return fail(hipErrorInvalidValue, "shape table entry does not hold its subarray length");
     t[Lr] = (unsigned char)sh;
   }
   {

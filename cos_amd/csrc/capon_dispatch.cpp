@@ -12,7 +12,6 @@ static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
   // one step of iterative refinement: its own kernels, with the fixed window bound when
   // no reference option needs the runtime-window kernels (nor a stored covariance)
   if (a.refine) {
-    if constexpr (CPL == 3) return hipErrorInvalidValue;   // no refinement kernel for this shape
     if (TW <= 5 && !(a.fb || a.zmode || a.pmode || a.Rin))
       return launch_one<float, G, CPL, 5, false, false, 0, true>(a, Mmax, s);
     return launch_one<float, G, CPL, 0, false, false, 0, true>(a, Mmax, s);
@@ -29,19 +28,14 @@ hipError_t launch_capon_f32_shape(const CaponLaunch& a, int shape, int Mmax, hip
     case 1: return launch_f32_tm<16, 1>(a, Mmax, s);
     case 2: return launch_f32_tm<16, 2>(a, Mmax, s);
     case 3: return launch_f32_tm<64, 1>(a, Mmax, s);
-    case 4:
-      if (a.refine) {
-        if (why) *why = "lane shape 4 (16 lanes x 3 columns) has no refinement kernel";
-        return hipErrorInvalidValue;
-      }
-      return launch_f32_tm<16, 3>(a, Mmax, s);
+    case 4: return launch_f32_tm<16, 3>(a, Mmax, s);
   }
   if (why) *why = "unknown lane shape";
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why) {
-  if (Lmax <= 64) return launch_capon_f32_shape(a, default_shape(Lmax, a.refine != 0), Mmax, s, why);
+  if (Lmax <= 64) return launch_capon_f32_shape(a, default_shape(Lmax), Mmax, s, why);
   if (why) *why = "L > 64 is not supported by the fused kernel";
   return hipErrorInvalidValue;
 }

@@ -771,10 +771,12 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
               // (S y)_c += x_t[k + c] beta_{t,k} for the G subarrays of the block
               static_for<0, G>([&](auto l2_) {
                 constexpr int l2 = decltype(l2_)::value;
-                if constexpr (CPL == 2)
+                if constexpr (CPL >= 2)
                   bcast_sub_conj_mul2<G, l2, l2 == 0>(sy[0], sy[1], nb, xv[l2], xv[l2 + G]);
                 else
                   bcast_sub_conj_mul<G, l2, l2 == 0>(sy[0], nb, xv[l2]);
+                if constexpr (CPL == 3)   // after the pair's FMAs, which do not write nb
+                  bcast_sub_conj_mul<G, l2, false>(sy[2], nb, xv[l2 + 2 * G]);
               });
             }
           }

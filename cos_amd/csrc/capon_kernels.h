@@ -55,15 +55,15 @@ hipError_t launch_cov_mfma(const CaponLaunch& a, int ntl, int Mmax, int Rs, hipS
 
 // Lane shapes of the FP32 fused kernel, the autotuner's choices per subarray length:
 // 1 = 16 lanes x 1 column (L <= 16), 2 = 16 lanes x 2 columns (L <= 32), 3 = 64 lanes
-// (L <= 64), 4 = 16 lanes x 3 columns (L <= 48; no refinement kernel).  default_shape is
+// (L <= 64), 4 = 16 lanes x 3 columns (L <= 48).  default_shape is
 // the 16-lane shape with the fewest columns that holds L (four pixels per wave beat the
 // 64-lane shape's one wherever a 16-lane group holds L), else the 64-lane shape.
 constexpr int kShapes = 4;
 inline int shape_cap(int shape) {
   return shape == 1 ? 16 : shape == 2 ? 32 : shape == 3 ? 64 : shape == 4 ? 48 : 0;
 }
-inline bool shape_fits(int shape, int L, bool refine) { return shape_cap(shape) >= L && !(refine && shape == 4); }
-inline int default_shape(int L, bool refine) { return L <= 16 ? 1 : L <= 32 ? 2 : L <= 48 && !refine ? 4 : 3; }
+inline bool shape_fits(int shape, int L) { return shape_cap(shape) >= L; }
+inline int default_shape(int L) { return L <= 16 ? 1 : L <= 32 ? 2 : L <= 48 ? 4 : 3; }
 
 // Returns hipSuccess or an error; `why` receives a static message on argument errors.
 hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);

@@ -136,7 +136,7 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
  * reference has one kernel configuration).  For each subarray length the frame uses (the
  * uniform L, or each distinct value of the row table) it times every lane shape that holds
  * that length (1: 16 lanes x 1 column, L <= 16; 2: 16 lanes x 2 columns, L <= 32;
- * 3: 64 lanes, L <= 64; 4: 16 lanes x 3 columns, L <= 48, not with refine) on up to 128
+ * 3: 64 lanes, L <= 64; 4: 16 lanes x 3 columns, L <= 48) on up to 128
  * output rows of that length (all beams) and keeps the fastest for this device and frame
  * configuration (Mmax, Yavg, options, refine); later cos_capon_run calls on the device use
  * it.  P, z, w (device, nullable, as cos_capon_run) receive the timed rows' results.
@@ -149,8 +149,7 @@ int cos_capon_autotune(cos_capon_plan* plan, const cos_capon_desc* desc, const c
 
 /* Installs a lane-shape table shapes[65] (0 = the smallest shape that holds L) for the
  * current device and the frame configuration of desc, e.g. one saved from
- * cos_capon_autotune.  Rejects a shape that does not hold its L, and shape 4 when
- * desc->refine is set. */
+ * cos_capon_autotune.  Rejects a shape that does not hold its L. */
 int cos_capon_tuning_set(const cos_capon_desc* desc, const int* shapes);
 
 /* Beamspace Capon with the sliding DFT: replaces the reference's SLIDING_BEAMSPACE build

@@ -155,8 +155,9 @@ def test_refinement_on_ill_conditioned_covariance():
 @pytest.mark.parametrize("M,L,Y", [(24, 12, 1), (48, 24, 2), (80, 40, 1), (128, 64, 2), (16, 1, 2), (32, 32, 1),
                                    (20, 20, 0)])
 def test_refined_shapes_match_oracle(M, L, Y):
-    """Every lane shape of the refined kernel (16 lanes x 1 or 2 columns, 64 lanes) against
-    the FP64 oracle: P within 1e-4, w within 1e-3 of its largest entry, z within 1e-4."""
+    """The refined kernel in the default lane shape of each L (16 lanes x 1, 2 or 3 columns,
+    64 lanes) against the FP64 oracle: P within 1e-4, w within 1e-3 of its largest entry,
+    z within 1e-4."""
     say("refine shape", M, L, Y)
     x = uniform_cube(3 * M + L, 3, 14, M)
     o, _ = _plan_run(x, 0.05, L, Y, refine=True)
@@ -288,6 +289,37 @@ def test_every_lane_shape_matches_oracle(M, L, Y):
             np.testing.assert_allclose(o["z"].cpu().numpy(), ref["z"].ravel(), atol=1e-4 * np.abs(ref["z"]).max())
     finally:
         CaponPlan.set_tuning(xt, 0.05, L, Y, {})
+
+
+@pytest.mark.parametrize("M,L,Y", [(80, 40, 1), (96, 48, 4), (72, 36, 2)])
+def test_three_column_shape_refined(M, L, Y):
+    """Refinement in lane shape 4 (the fixed five-row and runtime-window kernels), forced
+    through the tuning table, against the FP64 oracle and against the same frame refined
+    by the 64-lane shape: P within 1e-4 (and at least as close on average as unrefined),
+    w within 1e-3 of its largest entry, z within 1e-4."""
+    import torch
+    from cos_amd import CaponPlan
+    say("shape 4 refined", M, L, Y)
+    x = uniform_cube(2 * M + L + Y, 3, 14, M)
+    xt = torch.from_numpy(x).cuda()
+    ref = O.capon(x.astype(np.complex128), 0.05, L, Y)
+    out = {}
+    try:
+        for sh in (3, 4):
+            for refine in (False, True):
+                CaponPlan.set_tuning(xt, 0.05, L, Y, {L: sh}, refine=refine)
+                out[sh, refine], _ = _plan_run(x, 0.05, L, Y, refine=refine)
+    finally:
+        for refine in (False, True):
+            CaponPlan.set_tuning(xt, 0.05, L, Y, {}, refine=refine)
+    e = {k: rel(o["P"], ref["P"].ravel()) for k, o in out.items()}
+    print({k: f"{v.max():.2e}/{v.mean():.2e}" for k, v in e.items()}, flush=True)
+    o = out[4, True]
+    assert e[4, True].max() <= 1e-4
+    assert e[4, True].mean() <= e[4, False].mean()
+    assert e[4, True].mean() <= 2 * e[3, True].mean() + 1e-9
+    assert np.abs(o["z"] - ref["z"].ravel()).max() <= 1e-4 * np.abs(ref["z"]).max()
+    np.testing.assert_allclose(o["w"], ref["w"].reshape(-1, L), atol=1e-3 * np.abs(ref["w"]).max())
 
 
 @pytest.mark.parametrize("case", ["options", "steering", "mfma_covariance"])
