@@ -130,6 +130,10 @@ struct cos_capon_plan {
   std::vector<RowRun> runs;   // lane-shape runs of output rows for table-driven frames
   TuneTable shapes{};         // the tuned shapes the current frame uses
   bool fallback_ran = false;
+  // pixels the condition bound flags in the current frame (host count from the L and d
+  // tables): the FP64 fallback runs one block per group over this many list entries plus
+  // a slack, and a bounded persistent grid over any entries past them
+  long fb_bound = 0;
   ~cos_capon_plan() {
     fg.reset();
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -350,11 +354,25 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
   if (!fb) return 0;
   e = cosk::compact_flags(fl, a.npix, plan->list.as<int>(), plan->count.as<int>(), s);
   if (e != hipSuccess) return fail_hip(e, "compaction");
+  // The FP64 pass over the device-side list: the entries the host expects (the bound's
+  // count plus a slack for FP32 breakdowns) one block per group, the out-of-line
+  // persistent kernel only over entries past them (none in the usual frame).  The split
+  // changes only the launch shapes: every list entry is covered whatever the count.
+  constexpr long kSlack = 4096;
+  const int head = (int)std::min<long>(a.npix, plan->fb_bound + kSlack);
   cosk::CaponLaunch f = a;
   f.flags = nullptr;
   f.plist = plan->list.as<int>();
   f.pcount = plan->count.as<int>();
-  e = cosk::launch_capon_f64(f, Lmax, Mmax, s, &why);
+  f.pbase = 0;
+  f.npix = head;
+  e = cosk::launch_capon_f64_list(f, Lmax, Mmax, false, s, &why);
+  if (e == hipSuccess && head < a.npix) {
+    f.plist += head;
+    f.pbase = head;
+    f.npix = a.npix - head;
+    e = cosk::launch_capon_f64_list(f, Lmax, Mmax, true, s, &why);
+  }
   if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 fallback");
   return 0;
 }
@@ -397,16 +415,31 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   plan->runs.clear();
   plan->shapes = tuned_shapes(ds);
   const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  if (!tables) {
+    const float th = (float)(ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6);
+    plan->fb_bound = (1.f + (float)ds.L / ds.d) > th ? npix : 0;
+  }
   if (tables && !replay) {
     // Read the L table (Nz ints) to split the rows into runs of one lane shape (the tuned
     // one, or the smallest that holds L); more than 16 runs fall back to a single launch
     // sized for Lmax.  (ordered after the caller's work on s, which may still be writing
     // the table)
-    std::vector<int> hL(ds.Nz);
-    if ((e = hipMemcpyAsync(hL.data(), ds.row_L, sizeof(int) * ds.Nz, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+    std::vector<int> hL(ds.Nz, ds.L);
+    std::vector<float> hd(ds.Nz, ds.d);
+    if ((ds.row_L &&
+         (e = hipMemcpyAsync(hL.data(), ds.row_L, sizeof(int) * ds.Nz, hipMemcpyDeviceToHost, s)) != hipSuccess) ||
+        (fb && ds.row_d &&
+         (e = hipMemcpyAsync(hd.data(), ds.row_d, sizeof(float) * ds.Nz, hipMemcpyDeviceToHost, s)) != hipSuccess) ||
         (e = hipStreamSynchronize(s)) != hipSuccess)
       return fail_hip(e, "read L table");
     const int Nzp = ds.Nz - 2 * ds.Yavg;
+    if (fb) {   // the kernel's bound test, (1 + L/d) > cond_thresh in FP32, per centre row
+      const float th = (float)(ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6);
+      long rows = 0;
+      for (int jj = 0; jj < Nzp; ++jj)
+        rows += (1.f + (float)hL[jj + ds.Yavg] / hd[jj + ds.Yavg]) > th ? 1 : 0;
+      plan->fb_bound = rows * ds.Nx;
+    }
     for (int jj = 0; jj < Nzp;) {
       const int sh = shape_for(plan->shapes, hL[jj + ds.Yavg]);
       int j2 = jj + 1;

@@ -40,12 +40,11 @@ hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why) {
+static hipError_t f64_launch(const CaponLaunch& a, int Lmax, int Mmax, bool list, hipStream_t s, const char** why) {
   if (a.nsteer > 0) {
     if (why) *why = "steering vectors are not supported on the FP64 path";
     return hipErrorInvalidValue;
   }
-  const bool list = a.pcount != nullptr;
   if (Lmax <= 16)
     return list ? launch_one<double, 16, 1, 0, true, true, false>(a, Mmax, s) : launch_one<double, 16, 1, 0, true, false, false>(a, Mmax, s);
   if (Lmax <= 32)
@@ -54,6 +53,16 @@ hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_
     return list ? launch_one<double, 64, 1, 0, false, true, false>(a, Mmax, s) : launch_one<double, 64, 1, 0, false, false, false>(a, Mmax, s);
   if (why) *why = "L > 64 is not supported by the FP64 kernel";
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why) {
+  return f64_launch(a, Lmax, Mmax, a.pcount != nullptr, s, why);
+}
+
+hipError_t launch_capon_f64_list(const CaponLaunch& a, int Lmax, int Mmax, bool persist, hipStream_t s,
+                                 const char** why) {
+  if (!a.plist || !a.pcount) return hipErrorInvalidValue;
+  return f64_launch(a, Lmax, Mmax, persist, s, why);
 }
 
 hipError_t launch_capon_bwd(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why) {

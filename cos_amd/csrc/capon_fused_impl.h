@@ -1033,7 +1033,7 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   // list (the FP64 fallback) can use a bounded grid; otherwise one block per group (a
   // loop costs the full-frame kernels registers).
   constexpr int NGk = Grp<G>::NG;
-  const int npix = a.pcount ? min(*a.pcount, a.npix) : a.npix;
+  const int npix = a.pcount ? min(*a.pcount - a.pbase, a.npix) : a.npix;
   if constexpr (PERSIST) {
     for (int gid0 = blockIdx.x * NGk; gid0 < npix; gid0 += gridDim.x * NGk) {
       if constexpr (G == 64)

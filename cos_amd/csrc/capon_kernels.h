@@ -20,6 +20,8 @@ struct CaponLaunch {
   const float* rowD;
   const int* plist;    // optional pixel list (values p); null = all pixels
   const int* pcount;   // optional device count for plist (overrides npix when set)
+  int pbase;           // with pcount: list entries before plist; the launch covers entries
+                       // pbase .. *pcount-1 of the whole list, at most npix of them
   int npix;            // number of pixels (list capacity when pcount is set)
   int jlo, jn;         // without plist: output rows jlo .. jlo+jn-1 of every beam (npix = Nx*jn)
   float* P;            // [npix_total] power 1/(a^H R_L^-1 a), R normalised by K(2Y+1)
@@ -69,6 +71,11 @@ inline int default_shape(int L) { return L <= 16 ? 1 : L <= 32 ? 2 : L <= 48 ? 4
 hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
 hipError_t launch_capon_f32_shape(const CaponLaunch& a, int shape, int Mmax, hipStream_t s, const char** why);
 hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
+// The FP64 kernel over a device-side pixel list (a.plist, a.pcount, a.pbase): one block per
+// group of list entries over a grid sized by a.npix (persist = false; entries past the
+// count exit at once), or a bounded persistent grid (persist = true).
+hipError_t launch_capon_f64_list(const CaponLaunch& a, int Lmax, int Mmax, bool persist, hipStream_t s,
+                                 const char** why);
 hipError_t launch_capon_bwd(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
 
 }  // namespace cosk
