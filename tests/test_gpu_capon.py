@@ -414,6 +414,28 @@ def test_autotune_aperture_rows():
 
 # ------------------------------------------------------------------ per-row apertures and fallback
 
+def test_fallback_list_past_the_expected_count():
+    """More flagged pixels than the host expects (FP32 breakdowns on rank-deficient windows,
+    none flagged by the condition bound): the list entries past the expected count go to
+    the persistent FP64 grid, and every flagged pixel matches the all-FP64 frame, which
+    runs the same FP64 kernel without a list."""
+    M, L, Y, d = 33, 32, 0, 1e-12   # two subarrays: S_L has rank 2 plus a loading of 1e-12
+    say("fallback tail", 8, 2048, M, L, Y, d)
+    x = uniform_cube(33, 8, 2048, M)
+    o, plan = _plan_run(x, d, L, Y, precision="fp32+fallback", cond_thresh=1e30, want=("P", "z", "w", "flags"))
+    n = plan.fallback_count()
+    flagged = np.nonzero(o["flags"])[0]
+    print(f"flagged {n} of {o['P'].size}", flush=True)
+    assert n == len(flagged)
+    assert n > 4096 + 1000          # past the head launch (host count 0 + 4096)
+    assert np.all((o["flags"][flagged] & 2) != 0)   # breakdowns, not the bound
+    ref, _ = _plan_run(x, d, L, Y, precision="fp64", want=("P", "z", "w"))
+    assert np.all(np.isfinite(o["P"][flagged]))
+    np.testing.assert_allclose(o["P"][flagged], ref["P"][flagged], rtol=1e-6, atol=0)
+    np.testing.assert_allclose(o["w"][flagged], ref["w"][flagged], rtol=0, atol=1e-6 * np.abs(ref["w"][flagged]).max())
+    np.testing.assert_allclose(o["z"][flagged], ref["z"][flagged], rtol=0, atol=1e-6 * np.abs(ref["z"][flagged]).max())
+
+
 def test_rows_tables_and_fp64_fallback():
     import torch
     from cos_amd import synth
