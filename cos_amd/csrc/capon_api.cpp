@@ -68,8 +68,8 @@ struct RowRun {
 
 // Autotuned lane shapes: for each device and each frame configuration that changes the
 // kernels' cost (largest aperture, window, options), the fastest shape per subarray
-// length L = 1..64 as measured by cos_capon_autotune (0 = not tuned: the smallest shape
-// that holds L).  Process-wide, so every plan on the device uses it; g_tune_gen counts
+// length L = 1..64 as measured by cos_capon_autotune (0 = not tuned: default_shape,
+// the 16-lane shape with the fewest columns that holds L, 64 lanes past L = 48).  Process-wide, so every plan on the device uses it; g_tune_gen counts
 // changes so captured frame graphs are re-captured after one.
 using TuneKey = std::tuple<int, int, int, int, int, int, int>;   // device, Mmax, Yavg, fb, z, p, refine
 using TuneTable = std::array<unsigned char, 65>;
@@ -421,9 +421,9 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   }
   if (tables && !replay) {
     // Read the L table (Nz ints) to split the rows into runs of one lane shape (the tuned
-    // one, or the smallest that holds L); more than 16 runs fall back to a single launch
-    // sized for Lmax.  (ordered after the caller's work on s, which may still be writing
-    // the table)
+    // one, or default_shape); more than 16 runs fall back to a single launch sized for
+    // Lmax.  With the fallback on, the d table too, for the bound's pixel count.  (ordered
+    // after the caller's work on s, which may still be writing the tables)
     std::vector<int> hL(ds.Nz, ds.L);
     std::vector<float> hd(ds.Nz, ds.d);
     if ((ds.row_L &&

@@ -147,9 +147,10 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
 int cos_capon_autotune(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x, float* P,
                        cos_cf32* z, cos_cf32* w, int wstride, void* stream, int* shapes_out, float* ms_out);
 
-/* Installs a lane-shape table shapes[65] (0 = the smallest shape that holds L) for the
- * current device and the frame configuration of desc, e.g. one saved from
- * cos_capon_autotune.  Rejects a shape that does not hold its L. */
+/* Installs a lane-shape table shapes[65] for the current device and the frame
+ * configuration of desc, e.g. one saved from cos_capon_autotune.  0 = untuned: the 16-lane
+ * shape with the fewest columns that holds L, 64 lanes past L = 48.  Rejects a shape that
+ * does not hold its L. */
 int cos_capon_tuning_set(const cos_capon_desc* desc, const int* shapes);
 
 /* Beamspace Capon with the sliding DFT: replaces the reference's SLIDING_BEAMSPACE build
