@@ -38,6 +38,7 @@ DROPIN_CASES = [
     (32, 5, 0, 10, 3, 0.2), (32, 8, 1, 2, 57, 0.2), (32, 16, 1, 10, 57, 0.2),
     (32, 32, 0, 3, 9, 0.2), (16, 1, 2, 2, 9, 0.2), (64, 32, 2, 4, 40, 0.01),
     (40, 20, 2, 3, 30, 0.05), (96, 48, 2, 1, 12, 0.01), (128, 64, 2, 1, 9, 0.01),
+    (80, 37, 1, 2, 20, 0.05),   # 16 lanes x 3 columns with identity padding (37 < 48)
 ]
 
 
@@ -568,12 +569,13 @@ def test_dropin_beamspace_matrix_path(M, L, Nb):
 
 # ------------------------------------------------------------------ components
 
-def test_buildr_component_elementspace():
+@pytest.mark.parametrize("M,L", [(32, 8), (72, 37)])
+def test_buildr_component_elementspace(M, L):
     from cos_amd import BuildR
-    say("buildR", 32, 8)
-    x = uniform_cube(5, 3, 10, 32)
+    say("buildR", M, L)
+    x = uniform_cube(5, 3, 10, M)
     Nx, Nz, M = x.shape
-    L, Y, d = 8, 1, 0.2
+    Y, d = 1, 0.2
     R = np.zeros((Nx, Nz - 2 * Y, L, L), np.complex64)
     assert BuildR().getR(R, x, d, L, Y, M, Nx, Nz, 0) == 0
     ref = O.reference_gpu_outputs(x.astype(np.complex128), float(np.float32(d)), L, Y)["R"]
