@@ -39,11 +39,15 @@ int fail_hip(hipError_t e, const char* where) {
   return (int)e;
 }
 
+// Grow-only device buffer.  A reallocation bumps *gen (the owning plan's buffer
+// generation), so a frame graph that captured the old address is never replayed.
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
+  unsigned* gen = nullptr;
   hipError_t ensure(size_t bytes) {
     if (bytes <= n) return hipSuccess;
+    if (gen) ++*gen;
     if (p) hipFree(p);
     p = nullptr;
     n = 0;
@@ -101,13 +105,14 @@ inline int shape_for(const TuneTable& t, int L) {
 }
 
 // The frame's launches captured once and replayed: valid while the descriptor, every
-// pointer and wstride are unchanged.
+// pointer, wstride, the tuning table and the plan's own buffers are unchanged.
 struct FrameGraph {
   bool valid = false;
   cos_capon_desc ds;
   const void* ptr[7];
   int wstride = 0;
   int tune_gen = 0;
+  unsigned buf_gen = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   void reset() {
@@ -134,6 +139,10 @@ struct cos_capon_plan {
   // tables): the FP64 fallback runs one block per group over this many list entries plus
   // a slack, and a bounded persistent grid over any entries past them
   long fb_bound = 0;
+  unsigned buf_gen = 0;       // bumped by every reallocation of a plan buffer
+  cos_capon_plan() {
+    for (DevBuf* b : {&flags, &list, &count, &x, &z, &w, &R, &P, &B, &bs_tw}) b->gen = &buf_gen;
+  }
   ~cos_capon_plan() {
     fg.reset();
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -411,7 +420,7 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   FrameGraph& g = plan->fg;
   const int gen = g_tune_gen.load();
   const bool replay = ds.graph && g.valid && same_desc(g.ds, ds) && std::memcmp(g.ptr, ptr, sizeof(ptr)) == 0 &&
-                      g.wstride == wstride && g.tune_gen == gen;
+                      g.wstride == wstride && g.tune_gen == gen && g.buf_gen == plan->buf_gen;
   plan->runs.clear();
   plan->shapes = tuned_shapes(ds);
   const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
@@ -475,6 +484,7 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
     std::memcpy(g.ptr, ptr, sizeof(ptr));
     g.wstride = wstride;
     g.tune_gen = gen;
+    g.buf_gen = plan->buf_gen;
     g.valid = true;
   }
   if ((e = hipGraphLaunch(g.exec, s)) != hipSuccess) return fail_hip(e, "graph launch");

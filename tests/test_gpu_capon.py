@@ -703,6 +703,43 @@ def test_graph_replay_matches_eager():
             assert torch.equal(o3[k], e2r[k]), (rep, k)
 
 
+def test_graph_replay_after_plan_buffers_grow():
+    """Graph of frame A (row tables, fallback on), then an eager larger frame B on the same
+    plan, whose fallback buffers (flags, list, count) must grow, then graph A again: the
+    second graph call re-captures onto the new buffers and matches eager A exactly, with
+    the fallback count of A."""
+    import torch
+    from cos_amd import CaponPlan, synth
+    say("graph replay after regrow")
+    def rows_for(Nz, d):
+        return {"L": torch.full((Nz,), 32, dtype=torch.int32).cuda(), "M": torch.full((Nz,), 64, dtype=torch.int32).cuda(),
+                "off": torch.zeros(Nz, dtype=torch.int32).cuda(), "d": torch.full((Nz,), d).cuda()}
+    xa = torch.from_numpy(synth.config2(Nx=4, Nz=128, M=64, seed=2)).cuda()
+    xb = torch.from_numpy(synth.config2(Nx=16, Nz=512, M=64, seed=3)).cuda()
+    ra, rb = rows_for(128, 1e-6), rows_for(512, 1e-6)
+    plan = CaponPlan()
+    ea = plan.run(xa, 0.0, 0, 2, rows=ra, precision="fp32+fallback")
+    torch.cuda.synchronize()
+    na = plan.fallback_count()
+    ea = {k: v.clone() for k, v in ea.items()}
+    oa = {k: torch.empty_like(v) for k, v in ea.items()}
+    plan.run(xa, 0.0, 0, 2, rows=ra, precision="fp32+fallback", out=oa, graph=True)
+    torch.cuda.synchronize()
+    for k in ea:
+        assert torch.equal(oa[k], ea[k]), ("first graph", k)
+    eb = plan.run(xb, 0.0, 0, 2, rows=rb, precision="fp32+fallback")     # grows the plan's buffers
+    torch.cuda.synchronize()
+    assert plan.fallback_count() == 16 * 508
+    for v in oa.values():
+        v.zero_()
+    plan.run(xa, 0.0, 0, 2, rows=ra, precision="fp32+fallback", out=oa, graph=True)
+    torch.cuda.synchronize()
+    assert plan.fallback_count() == na == 4 * 124
+    for k in ea:
+        assert torch.equal(oa[k], ea[k]), ("graph after regrow", k)
+    del eb
+
+
 # ------------------------------------------------------------------ sliding-DFT beamspace
 
 @pytest.mark.parametrize("M,L,Y,Nb,d,Nz", [(12, 6, 1, 2, 0.2, 40), (32, 16, 2, 5, 0.05, 40), (20, 9, 0, 1, 0.1, 40),
