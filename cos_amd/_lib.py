@@ -16,7 +16,7 @@ EXPORTS = (
     "cos_capon_last_error", "cos_capon_plan_create", "cos_capon_plan_destroy", "cos_capon_run",
     "cos_capon_last_fallback_count", "cos_capon_beamspace_run", "cos_capon_backward",
     "cos_capon_subspace_run", "cos_tdib_run", "cos_lca_run", "cos_capon_autotune",
-    "cos_capon_tuning_set",
+    "cos_capon_tuning_set", "cos_capon_run_noalloc",
 )
 
 COS_PREC_FP32 = 0
@@ -74,6 +74,7 @@ def lib() -> C.CDLL:
     L.cos_capon_plan_destroy.argtypes = [vp]
     L.cos_capon_run.argtypes = [vp, C.POINTER(CaponDesc), vp, vp, vp, vp, ip, vp, vp, vp]
     L.cos_capon_last_fallback_count.argtypes = [vp, C.POINTER(C.c_int)]
+    L.cos_capon_run_noalloc.argtypes = [C.POINTER(CaponDesc), vp, vp, vp, vp, ip, vp, vp]
     L.cos_capon_beamspace_run.argtypes = [vp, C.POINTER(CaponDesc), ip, vp, vp, vp, vp, vp, ip, vp]
     L.cos_capon_backward.argtypes = [vp, C.POINTER(CaponDesc), vp, vp, vp, vp, vp]
     L.cos_capon_subspace_run.argtypes = [vp, C.POINTER(CaponDesc), vp, ip, vp, vp, vp, vp, vp]

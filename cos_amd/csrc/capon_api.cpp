@@ -243,8 +243,31 @@ int cos_capon_plan_destroy(cos_capon_plan* plan) {
 
 namespace {
 
-// Enqueue one frame on stream s (arguments already validated; buffers already sized).
-int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32* x, float* P, cos_cf32* z,
+// What one frame's launches need beyond the descriptor and the caller's buffers: the
+// lane-shape runs of a table-driven frame, the tuned shapes, the host's count of pixels
+// the condition bound flags, and device scratch (per-pixel flags when the caller passes
+// none, the compacted FP64 pixel list and its device-side length, the matrix-core
+// covariance when the caller passes no R).
+struct FrameCtx {
+  const RowRun* runs = nullptr;
+  int nruns = 0;
+  TuneTable shapes{};
+  long fb_bound = 0;
+  uint8_t* flags = nullptr;
+  int* list = nullptr;
+  int* count = nullptr;
+  float2* R = nullptr;
+};
+
+struct RunSpan {
+  const RowRun *b, *e;
+  const RowRun* begin() const { return b; }
+  const RowRun* end() const { return e; }
+};
+inline RunSpan ctx_runs(const FrameCtx& c) { return {c.runs, c.runs + c.nruns}; }
+
+// Enqueue one frame on stream s (arguments already validated; scratch already sized).
+int enqueue_frame(const FrameCtx& c, const cos_capon_desc& ds, const cos_cf32* x, float* P, cos_cf32* z,
                   cos_cf32* w, int wstride, cos_cf32* R, unsigned char* flags, hipStream_t s, bool fb_needed);
 
 int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
@@ -289,7 +312,7 @@ bool fallback_needed(const cos_capon_desc& ds) {
   return bound > thresh || bound * ds.L * 5.9604645e-8 >= 0.01;
 }
 
-int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32* x, float* P, cos_cf32* z,
+int enqueue_frame(const FrameCtx& ctx, const cos_capon_desc& ds, const cos_cf32* x, float* P, cos_cf32* z,
                   cos_cf32* w, int wstride, cos_cf32* R, unsigned char* flags, hipStream_t s, bool fb_needed) {
   const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
   const int Lmax = tables ? ds.Lmax : ds.L;
@@ -308,10 +331,10 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
     // covariance on the matrix cores into R (the caller's, or the plan's; Lmax^2 per pixel),
     // then the solve reads it back; one launch per row run, with one or four 32 x 32 tiles
     cosk::CaponLaunch c = a;
-    c.R = R ? (float2*)R : plan->R.as<float2>();
+    c.R = R ? (float2*)R : ctx.R;
     const int Rs = Lmax * Lmax;
-    if (tables && !plan->runs.empty()) {
-      for (const RowRun& r : plan->runs) {
+    if (tables && ctx.nruns) {
+      for (const RowRun& r : ctx_runs(ctx)) {
         c.jlo = r.jlo;
         c.jn = r.jn;
         c.npix = ds.Nx * r.jn;
@@ -327,8 +350,8 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
     a.Rstride = Rs;
   }
   if (ds.precision == COS_PREC_FP64) {
-    if (tables && !plan->runs.empty()) {
-      for (const RowRun& r : plan->runs) {
+    if (tables && ctx.nruns) {
+      for (const RowRun& r : ctx_runs(ctx)) {
         cosk::CaponLaunch b = a;
         b.jlo = r.jlo;
         b.jn = r.jn;
@@ -344,11 +367,11 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
   }
   const bool fb = fb_needed;
   uint8_t* fl = (uint8_t*)flags;
-  if (fb && !fl) fl = plan->flags.as<uint8_t>();
+  if (fb && !fl) fl = ctx.flags;
   a.flags = fl;
-  if (tables && !plan->runs.empty()) {
+  if (tables && ctx.nruns) {
     // range rows grouped into runs of one L bucket, each launched with its lane shape
-    for (const RowRun& r : plan->runs) {
+    for (const RowRun& r : ctx_runs(ctx)) {
       cosk::CaponLaunch b = a;
       b.jlo = r.jlo;
       b.jn = r.jn;
@@ -357,22 +380,22 @@ int enqueue_frame(cos_capon_plan* plan, const cos_capon_desc& ds, const cos_cf32
       if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
     }
   } else {
-    e = cosk::launch_capon_f32_shape(a, shape_for(plan->shapes, Lmax), Mmax, s, &why);
+    e = cosk::launch_capon_f32_shape(a, shape_for(ctx.shapes, Lmax), Mmax, s, &why);
     if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
   }
   if (!fb) return 0;
-  e = cosk::compact_flags(fl, a.npix, plan->list.as<int>(), plan->count.as<int>(), s);
+  e = cosk::compact_flags(fl, a.npix, ctx.list, ctx.count, s);
   if (e != hipSuccess) return fail_hip(e, "compaction");
   // The FP64 pass over the device-side list: the entries the host expects (the bound's
   // count plus a slack for FP32 breakdowns) one block per group, the out-of-line
   // persistent kernel only over entries past them (none in the usual frame).  The split
   // changes only the launch shapes: every list entry is covered whatever the count.
   constexpr long kSlack = 4096;
-  const int head = (int)std::min<long>(a.npix, plan->fb_bound + kSlack);
+  const int head = (int)std::min<long>(a.npix, ctx.fb_bound + kSlack);
   cosk::CaponLaunch f = a;
   f.flags = nullptr;
-  f.plist = plan->list.as<int>();
-  f.pcount = plan->count.as<int>();
+  f.plist = ctx.list;
+  f.pcount = ctx.count;
   f.pbase = 0;
   f.npix = head;
   e = cosk::launch_capon_f64_list(f, Lmax, Mmax, false, s, &why);
@@ -459,7 +482,16 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
     }
     if (plan->runs.size() > 16) plan->runs.clear();
   }
-  if (!ds.graph) return enqueue_frame(plan, ds, x, P, z, w, wstride, R, flags, s, fb);
+  FrameCtx ctx;
+  ctx.runs = plan->runs.data();
+  ctx.nruns = (int)plan->runs.size();
+  ctx.shapes = plan->shapes;
+  ctx.fb_bound = plan->fb_bound;
+  ctx.flags = plan->flags.as<uint8_t>();
+  ctx.list = plan->list.as<int>();
+  ctx.count = plan->count.as<int>();
+  ctx.R = plan->R.as<float2>();
+  if (!ds.graph) return enqueue_frame(ctx, ds, x, P, z, w, wstride, R, flags, s, fb);
 
   if (!replay) {
     g.reset();
@@ -467,7 +499,7 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
       return fail_hip(e, "capture stream");
     if ((e = hipStreamBeginCapture(plan->capture_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return fail_hip(e, "begin capture");
-    const int r = enqueue_frame(plan, ds, x, P, z, w, wstride, R, flags, plan->capture_stream, fb);
+    const int r = enqueue_frame(ctx, ds, x, P, z, w, wstride, R, flags, plan->capture_stream, fb);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(plan->capture_stream, &graph);
     if (r) {
@@ -489,6 +521,81 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   }
   if ((e = hipGraphLaunch(g.exec, s)) != hipSuccess) return fail_hip(e, "graph launch");
   return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Entry point without allocation: the FP64 fallback's scratch (per-pixel flags when the
+// caller passes none, the compacted pixel list, its device-side length) lives in one
+// file-scope device array of fixed size instead of plan buffers.  Each device holds its
+// own copy until the process ends; host code uses only the address hipGetSymbolAddress
+// returns for the current device (the array's name is a host-side stand-in).  The count
+// is cleared by a fill kernel at the start of every compaction, so no call depends on
+// what an earlier call left behind; calls that overlap on one device share the array and
+// race.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kScratchCountBytes = 256;   // the list length, padded to the array's alignment
+constexpr size_t kScratchBytes =
+    kScratchCountBytes + (size_t)COS_CAPON_NOALLOC_MAX_PIXELS * (sizeof(int) + sizeof(uint8_t));
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current device's scratch array, looked up once per device.
+hipError_t scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cos_capon_run_noalloc(const cos_capon_desc* desc, const cos_cf32* x, float* P, cos_cf32* z, cos_cf32* w,
+                          int wstride, unsigned char* flags, void* stream) {
+  g_last_error.clear();
+  if (!desc || !x) return fail(hipErrorInvalidValue, "desc and x must be non-NULL");
+  const cos_capon_desc ds = *desc;
+  if (ds.row_L || ds.row_M || ds.row_off || ds.row_d)
+    return fail(hipErrorInvalidValue, "the no-allocation entry point takes a uniform L, M and d (no row tables)");
+  if (ds.graph) return fail(hipErrorInvalidValue, "the no-allocation entry point does not capture graphs");
+  if (ds.cov_mfma) return fail(hipErrorInvalidValue, "the no-allocation entry point has no matrix-core covariance");
+  if (int e = validate(ds, nullptr, w, wstride)) return e;
+  const long npix = (long)ds.Nx * (ds.Nz - 2 * ds.Yavg);
+  if (npix > COS_CAPON_NOALLOC_MAX_PIXELS)
+    return fail(hipErrorInvalidValue, "the no-allocation entry point takes at most COS_CAPON_NOALLOC_MAX_PIXELS "
+                                      "(2097152) output pixels per call");
+  const bool fb = fallback_needed(ds);
+  FrameCtx ctx;
+  ctx.shapes = tuned_shapes(ds);
+  const float th = (float)(ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6);
+  ctx.fb_bound = (1.f + (float)ds.L / ds.d) > th ? npix : 0;
+  if (fb) {
+    void* base = nullptr;
+    if (hipError_t e = scratch_address(&base)) return fail_hip(e, "scratch address");
+    unsigned char* b = static_cast<unsigned char*>(base);
+    ctx.count = reinterpret_cast<int*>(b);
+    ctx.list = reinterpret_cast<int*>(b + kScratchCountBytes);
+    ctx.flags = b + kScratchCountBytes + (size_t)COS_CAPON_NOALLOC_MAX_PIXELS * sizeof(int);
+  }
+  return enqueue_frame(ctx, ds, x, P, z, w, wstride, nullptr, flags, (hipStream_t)stream, fb);
 }
 
 int cos_capon_autotune(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf32* x, float* P,

@@ -132,6 +132,19 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
                   float* P, cos_cf32* z, cos_cf32* w, int wstride, cos_cf32* R,
                   unsigned char* flags, void* stream);
 
+/* As cos_capon_run with no plan and no allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The FP64 fallback's scratch (flags when the
+ * caller passes none, the pixel list and its count) is a fixed device array of
+ * 256 + 5 * COS_CAPON_NOALLOC_MAX_PIXELS bytes per GPU, held until the process ends, so a
+ * call takes at most COS_CAPON_NOALLOC_MAX_PIXELS output pixels (larger: error 1).
+ * Uniform L, M and d only (no row tables), no graph capture, no matrix-core covariance,
+ * no R output; every other descriptor field as cos_capon_run.  Calls that overlap on one
+ * GPU share its scratch and race: serialise them (one stream, or one call at a time). */
+#define COS_CAPON_NOALLOC_MAX_PIXELS 2097152
+int cos_capon_run_noalloc(const cos_capon_desc* desc, const cos_cf32* x, float* P, cos_cf32* z,
+                          cos_cf32* w, int wstride, unsigned char* flags, void* stream);
+
 /* Autotuner of the fused FP32 kernel's lane shape (no reference counterpart; the
  * reference has one kernel configuration).  For each subarray length the frame uses (the
  * uniform L, or each distinct value of the row table) it times every lane shape that holds
