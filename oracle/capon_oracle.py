@@ -338,8 +338,11 @@ def capon_beamspace_matrix(x: np.ndarray, d: float, L: int, Yavg: int, Nb: int, 
     loaded unnormalised S_L, R_bs = B S_L B^H, a_bs = B 1, y_bs = R_bs^-1 a_bs,
     y = B^H y_bs, then the element-space output stage w = y/sum(y), z = w^H g.
     P (not produced by the reference) is 1/(a_bs^H R_bs'^-1 a_bs) for the normalised
-    R_bs' = R_bs/(K(2Y+1)).  parity unpinned: third-party cublasCgemmBatched products,
-    whose only reference test is report-only (CAP/tests/testRTC.pyx:386-500)."""
+    R_bs' = R_bs/(K(2Y+1)).  The reference computes the products with cublasCgemmBatched,
+    whose only reference test is report-only (CAP/tests/testRTC.pyx:386-500); pinned in
+    tests/test_oracle.py at Nb < L by the sliding-DFT restatement (d = 0: same P, w, z;
+    d > 0: same with the per-pixel loading that element-space loading implies) and by the
+    closed form for S = c I."""
     Nx, Nz, M = x.shape
     K = M - L + 1
     if pixels is None:

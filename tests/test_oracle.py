@@ -250,6 +250,74 @@ def test_beamspace_full_basis_equals_element_space():
     np.testing.assert_allclose(m["w"], e["w"], atol=1e-11)
 
 
+def _beamspace_case(seed):
+    # Nb < L: B has orthonormal rows but is not unitary, so B S B^H loses information and
+    # the place where the loading is applied matters (K (2Y+1) = 27 snapshots >= Nb)
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((2, 9, 24)) + 1j * rng.standard_normal((2, 9, 24))
+    return x, 16, 1, 5
+
+
+def test_beamspace_matrix_unloaded_equals_sliding_beamspace():
+    """At d = 0 the reference's matrix path (B S B^H by products, a_bs = B 1, y = B^H y_bs,
+    then the element-space output stage, RTC/Capon.cpp:285-402) and the sliding-DFT path
+    (Gram of the beamspace subarrays, w = B^H w_bs, z = w_bs^H B g, RTC/Capon.cpp:202-227,
+    RTC/capon_kernel.cu:162-233) are two computations of one estimator: P, w and z agree."""
+    x, L, Y, Nb = _beamspace_case(21)
+    m = O.capon_beamspace_matrix(x, 0.0, L, Y, Nb)
+    b = O.capon_beamspace(x, 0.0, L, Y, Nb)
+    np.testing.assert_allclose(m["P"], b["P"], rtol=1e-10)
+    np.testing.assert_allclose(m["w"], b["w"], rtol=0, atol=1e-10 * np.abs(b["w"]).max())
+    np.testing.assert_allclose(m["z"], b["z"], rtol=0, atol=1e-10 * np.abs(b["z"]).max())
+    # distortionless in element space: a^H w = 1 (a = 1), the reference's w = y / sum(y)
+    np.testing.assert_allclose(m["w"].sum(axis=-1), 1.0, atol=1e-10)
+
+
+def test_beamspace_matrix_loads_in_element_space():
+    """At d > 0 the matrix path loads S_L = S + d tr(S)/L I before the products, so
+    R_bs = B S B^H + d tr(S)/L I_Nb (B B^H = I_Nb): the sliding-DFT path with the loading
+    d' = d (tr R_hat / L) / (tr R_bs / Nb) per pixel (its convention is d' tr(R_bs)/Nb,
+    CAP/getCaponAllC.pyx:376-385) must give the same P, w and z; with d itself it must not."""
+    x, L, Y, Nb = _beamspace_case(22)
+    d = 0.05
+    K = x.shape[-1] - L + 1
+    m = O.capon_beamspace_matrix(x, d, L, Y, Nb)
+    ix, j = O.all_pixels(x.shape[0], x.shape[1], Y)
+    rows = O.window_rows(x, ix, j, Y)
+    Rhat = O.covariance_unnormalised(rows, L) / (K * (2 * Y + 1))
+    v = O.sliding_dft(rows, L, Nb)                                     # beamspace subarrays
+    tr_bs = np.einsum("btkr,btkr->b", v, v.conj()).real / (K * (2 * Y + 1))
+    tr_es = np.trace(Rhat, axis1=-2, axis2=-1).real
+    dprime = d * (tr_es / L) / (tr_bs / Nb)
+    b = O.capon_beamspace(x, dprime, L, Y, Nb)
+    np.testing.assert_allclose(m["P"], b["P"], rtol=1e-10)
+    np.testing.assert_allclose(m["w"], b["w"], rtol=0, atol=1e-10 * np.abs(b["w"]).max())
+    np.testing.assert_allclose(m["z"], b["z"], rtol=0, atol=1e-10 * np.abs(b["z"]).max())
+    wrong = O.capon_beamspace(x, d, L, Y, Nb)
+    assert np.max(np.abs(wrong["P"] - m["P"]) / m["P"]) > 1e-3
+    # the materialised R is the element-space loaded S_L in the GPU layout
+    SL = O.diagonal_load(Rhat * (K * (2 * Y + 1)), d, L)
+    np.testing.assert_allclose(m["R"], np.conj(SL), rtol=0, atol=1e-12 * np.abs(SL).max())
+
+
+def test_beamspace_matrix_white_covariance_closed_form():
+    """Snapshots whose subarrays are orthogonal with equal energy give S = c I; then
+    R_bs = (c + d c) I_Nb in the matrix path and P = (1 + d) c / (norm L) since
+    a_bs = B 1 = sqrt(L) e_0: a closed form independent of both beamspace restatements."""
+    L, Nb, Y = 8, 3, 0
+    M = 2 * L - 1                        # K = L subarrays of one snapshot
+    # x[m] = 1 at m = L-1 only: subarray k holds e_{L-1-k}, so S = sum_k e e^H = I
+    x = np.zeros((1, 1, M), dtype=np.complex128)
+    x[0, 0, L - 1] = 1.0
+    d = 0.25
+    m = O.capon_beamspace_matrix(x, d, L, Y, Nb)
+    norm = L * (2 * Y + 1)
+    # S_L = (1 + d) I, R_bs = (1 + d) I, a_bs^H R_bs^-1 a_bs = L / (1 + d)
+    np.testing.assert_allclose(m["P"], (1 + d) / (L * norm), rtol=1e-12)
+    # y_bs = sqrt(L)/(1+d) e_0, y = B^H y_bs = (1/(1+d)) 1, w = 1/L
+    np.testing.assert_allclose(m["w"], np.full((1, L), 1.0 / L), atol=1e-14)
+
+
 # ---------------------------------------------------------------- steering / rows
 
 def test_steered_equals_rotated_dropin():
