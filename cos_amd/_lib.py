@@ -83,7 +83,7 @@ def lib() -> C.CDLL:
     L.cos_capon_tuning_set.argtypes = [C.POINTER(CaponDesc), C.POINTER(C.c_int)]
     db = C.c_double
     L.cos_tdib_run.argtypes = [vp, ip, ip, db, db, vp, vp, db, db, vp, vp, ip, ip, fl, vp, vp, vp]
-    L.cos_lca_run.argtypes = [vp, ip, ip, ip, vp, ip, ip, vp, vp, vp]
+    L.cos_lca_run.argtypes = [vp, ip, ip, ip, vp, ip, ip, ip, vp, vp, vp]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.cos_capon_last_error.restype = C.c_char_p

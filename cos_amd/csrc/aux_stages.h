@@ -27,6 +27,10 @@ struct LcaLaunch {
   int wy, wx;
   float2* img;             // [Nx][Ny] selected value (optional)
   int* win;                // [Nx][Ny] selected window (optional)
+  int branch;              // 0: centred neighbourhood; the reference's own branches:
+                           // 1 argmin |Z| (1 x 1 filter), 2 all-ones range filter
+                           // (sliding FP64 sums), 3 any other filter (swapped flat index)
+  int rows_per_chunk;      // branch 2: sliding rows whose sums are staged in LDS at once
 };
 
 hipError_t launch_tdib(const TdibLaunch& a, hipStream_t s);

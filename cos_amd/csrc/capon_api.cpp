@@ -770,13 +770,17 @@ int cos_tdib_run(const cos_cf32* mfdata, int n_h, int n_t, double t0, double dt,
   return e == hipSuccess ? 0 : fail_hip(e, "tdib kernel");
 }
 
-int cos_lca_run(const cos_cf32* Z, int Nx, int Ny, int Nw, const float* wrf, int wy, int wx,
+int cos_lca_run(const cos_cf32* Z, int Nx, int Ny, int Nw, const float* wrf, int wy, int wx, int mode,
                 cos_cf32* img, int* win, void* stream) {
   g_last_error.clear();
   if (!Z || !wrf) return fail(hipErrorInvalidValue, "NULL input");
   if (Nx < 1 || Ny < 1 || Nw < 1 || wy < 1 || wx < 1) return fail(hipErrorInvalidValue, "sizes must be >= 1");
   if (wy % 2 == 0 || wx % 2 == 0) return fail(hipErrorInvalidValue, "the wrf filter must have odd sizes");
+  if (mode != COS_LCA_REFERENCE && mode != COS_LCA_CENTRED) return fail(hipErrorInvalidValue, "bad LCA mode");
+  if ((long)Nx * Ny > (1L << 31) - 1 || (long)Nx * Ny * Nw > (1L << 40))
+    return fail(hipErrorInvalidValue, "too many pixels for one call");
   cosk::LcaLaunch a;
+  std::memset(&a, 0, sizeof(a));
   a.Z = (const float2*)Z;
   a.Nx = Nx;
   a.Ny = Ny;
@@ -786,7 +790,35 @@ int cos_lca_run(const cos_cf32* Z, int Nx, int Ny, int Nw, const float* wrf, int
   a.wx = wx;
   a.img = (float2*)img;
   a.win = win;
-  hipError_t e = cosk::launch_lca(a, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (mode == COS_LCA_REFERENCE) {
+    // the reference picks its branch from the filter's values (lcacx.pyx:94-100,122,139,185)
+    if ((long)wy * wx > 4096) return fail(hipErrorInvalidValue, "reference mode takes filters of at most 4096 taps");
+    float h[4096];
+    if ((e = hipMemcpyAsync(h, wrf, sizeof(float) * wy * wx, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+      return fail_hip(e, "read wrf");
+    bool ones = wx == 1;
+    for (int i = 0; i < wy * wx && ones; ++i) ones = h[i] == 1.0f;
+    const int Ym = wy / 2, Xm = wx / 2;
+    if (wy == 1 && wx == 1 && h[0] != 0.0f) {
+      a.branch = 1;
+    } else if (ones) {
+      if (Ny <= wy)
+        return fail(hipErrorInvalidValue, "reference mode: the all-ones range filter needs Ny > its length "
+                                          "(the reference writes row Y_AVG outside the image)");
+      if (Nw > 256) return fail(hipErrorInvalidValue, "reference mode: the all-ones range filter takes Nw <= 256");
+      a.branch = 2;
+      a.rows_per_chunk = std::max(1, std::min(64, 4096 / Nw));
+    } else {
+      if (Ny - 2 * Ym > 0 && Nx - 2 * Xm > 0 && Ny > Nx)
+        return fail(hipErrorInvalidValue, "reference mode: this filter needs Ny <= Nx (the reference's general "
+                                          "branch reads past its |Z|^2 array on taller images)");
+      a.branch = 3;
+    }
+  }
+  e = cosk::launch_lca(a, s);
   return e == hipSuccess ? 0 : fail_hip(e, "lca kernel");
 }
 

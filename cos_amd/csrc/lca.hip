@@ -8,10 +8,20 @@
 //   w*  = the first w with the smallest s_w,
 //   img[x][y] = Z[x][y][w*], win[x][y] = w*;
 // pixels nearer the edge get 0 (as in the reference, whose output arrays start at zero).
-// With a 1 x 1 filter this is the per-pixel argmin of |Z| (lcacx.pyx:119-125).  The
-// reference's general branch indexes its |Z|^2 array as [w][y][x] although it fills it as
-// [w][x][y] (lcacx.pyx:186-206), which only agrees for square images; this kernel uses the
-// intended [x][y] neighbourhood.
+// With a 1 x 1 filter this is the per-pixel argmin of |Z| (lcacx.pyx:119-125).  This is
+// the centred mode: the reference's general branch indexes its |Z|^2 array as [w][y][x]
+// although it fills it as [w][x][y] (lcacx.pyx:186-206), which only agrees for square
+// images, and its all-ones range branch never writes row Ym; the centred kernels use the
+// intended [x][y] neighbourhood at every interior pixel.
+//
+// Reference mode (lca_ref_* kernels) reproduces what lcacx.pyx computes, index for index,
+// in its three branches: the 1 x 1 argmin of |Z| (:122-127), the all-ones range filter
+// by FP64 sliding sums in its order (:139-182: rows 1+Ym .. Ny-Ym-1, row Y_AVG written
+// with window 0 by the first segment and overwritten unless it lies past the sliding
+// rows, the rest 0), and any other filter through the swapped flat index
+// w Nx Ny + (y+yy-Ym) Ny + (x+xx-Xm) into the |Z|^2 array filled as [w][x][y]
+// (:185-217).  Their FP64 arithmetic is uncontracted, as the reference's x86-64 build
+// (no FMA) rounds every product and sum on its own.
 //
 // lca_tile_kernel<NW> (NW <= 16 windows, Ym <= 32): a block owns one beam x and 256
 // ranges; for each beam of the filter's x extent it stages |Z|^2 of its 256 + 2 Ym rows
@@ -125,11 +135,159 @@ static hipError_t launch_tile_nw(const LcaLaunch& a, hipStream_t s) {
   }
 }
 
+// ---- reference mode ----
+
+// |z|^2 in FP64 (both products are exact for float inputs; the sum is rounded once)
+__device__ __forceinline__ double sq_f64(float2 v) {
+#pragma clang fp contract(off)
+  const double re = v.x, im = v.y;
+  return re * re + im * im;
+}
+
+// s + a b with the product rounded before the sum
+__device__ __forceinline__ double add_prod_f64(double s, double a, double b) {
+#pragma clang fp contract(off)
+  return s + a * b;
+}
+
+// s + (a - b), the difference rounded first (lcacx.pyx:171)
+__device__ __forceinline__ double add_diff_f64(double s, double a, double b) {
+#pragma clang fp contract(off)
+  return s + (a - b);
+}
+
+// 1 x 1 filter with a nonzero weight: the first window of smallest |Z| at every pixel.
+__global__ __launch_bounds__(256) void lca_ref_argmin_kernel(const LcaLaunch a) {
+  const long pix = (long)blockIdx.x * 256 + threadIdx.x;
+  if (pix >= (long)a.Nx * a.Ny) return;
+  const float2* zp = a.Z + pix * a.Nw;
+  double best = sq_f64(zp[0]);
+  int bw = 0;
+  for (int w = 1; w < a.Nw; ++w) {
+    const double v = sq_f64(zp[w]);
+    if (v < best) {
+      best = v;
+      bw = w;
+    }
+  }
+  if (a.img) a.img[pix] = zp[bw];
+  if (a.win) a.win[pix] = bw;
+}
+
+constexpr int LCA_REF_LANES_W = 4;   // windows per lane of the sliding kernel (Nw <= 256)
+
+// All-ones range filter: one wave per beam x, lane t keeps the running sums of windows
+// t, t+64, ...; each chunk of sliding rows stages its sums in LDS [rows][Nw] and the lanes
+// then take one row each for the first minimum over w.
+__global__ __launch_bounds__(64) void lca_ref_sliding_kernel(const LcaLaunch a) {
+  extern __shared__ double sb[];
+  const int x = blockIdx.x, t = threadIdx.x;
+  const int YA = a.wy, Ym = YA / 2, Ny = a.Ny, Nw = a.Nw, CH = a.rows_per_chunk;
+  const float2* Zx = a.Z + (size_t)x * Ny * Nw;
+  const size_t ob = (size_t)x * Ny;
+  const int ylo = 1 + Ym, yhi = Ny - Ym;   // rows of the sliding loop
+  // rows it does not write: 0 and window 0, except row Y_AVG when it lies past them,
+  // which keeps the first segment's window 0 (Y_AVG >= ylo always)
+  for (int y = t; y < Ny; y += 64) {
+    if (y >= ylo && y < yhi) continue;
+    const float2 v = y == YA ? Zx[(size_t)YA * Nw] : make_float2(0.f, 0.f);
+    if (a.img) a.img[ob + y] = v;
+    if (a.win) a.win[ob + y] = 0;
+  }
+  double s[LCA_REF_LANES_W];
+#pragma unroll
+  for (int i = 0; i < LCA_REF_LANES_W; ++i) s[i] = 0.0;
+  for (int y = 0; y < YA; ++y) {
+#pragma unroll
+    for (int i = 0; i < LCA_REF_LANES_W; ++i) {
+      const int w = t + 64 * i;
+      if (w < Nw) s[i] = s[i] + sq_f64(Zx[(size_t)y * Nw + w]);
+    }
+  }
+  for (int y0 = ylo; y0 < yhi; y0 += CH) {
+    const int nr = min(CH, yhi - y0);
+    for (int r = 0; r < nr; ++r) {
+      const int y = y0 + r;
+#pragma unroll
+      for (int i = 0; i < LCA_REF_LANES_W; ++i) {
+        const int w = t + 64 * i;
+        if (w < Nw) {
+          s[i] = add_diff_f64(s[i], sq_f64(Zx[(size_t)(y + Ym) * Nw + w]), sq_f64(Zx[(size_t)(y - Ym - 1) * Nw + w]));
+          sb[r * Nw + w] = s[i];
+        }
+      }
+    }
+    __syncthreads();
+    for (int r = t; r < nr; r += 64) {
+      const double* row = sb + r * Nw;
+      double best = row[0];
+      int bw = 0;
+      for (int w = 1; w < Nw; ++w)
+        if (row[w] < best) {
+          best = row[w];
+          bw = w;
+        }
+      const int y = y0 + r;
+      if (a.img) a.img[ob + y] = Zx[(size_t)y * Nw + bw];
+      if (a.win) a.win[ob + y] = bw;
+    }
+    __syncthreads();
+  }
+}
+
+// Any other filter: one thread per pixel; the |Z|^2 entry at flat index f of the
+// reference's [Nw][Nx][Ny] array is that of Z at pixel f mod (Nx Ny), window f / (Nx Ny).
+__global__ __launch_bounds__(256) void lca_ref_general_kernel(const LcaLaunch a) {
+  const long pix = (long)blockIdx.x * 256 + threadIdx.x;
+  if (pix >= (long)a.Nx * a.Ny) return;
+  const int x = (int)(pix / a.Ny), y = (int)(pix - (long)x * a.Ny);
+  const int Ym = a.wy / 2, Xm = a.wx / 2;
+  const long NP = (long)a.Nx * a.Ny;
+  float2 best_v = make_float2(0.f, 0.f);
+  int bw = 0;
+  if (y >= Ym && y < a.Ny - Ym && x >= Xm && x < a.Nx - Xm) {
+    double best = 0.0;
+    for (int w = 0; w < a.Nw; ++w) {
+      double sw = 0.0;
+      for (int yy = 0; yy < a.wy; ++yy)
+        for (int xx = 0; xx < a.wx; ++xx) {
+          const long f = (long)w * NP + (long)(y + yy - Ym) * a.Ny + (x + xx - Xm);
+          const long wq = f / NP, r = f - wq * NP;
+          sw = add_prod_f64(sw, sq_f64(a.Z[r * a.Nw + wq]), (double)a.wrf[yy * a.wx + xx]);
+        }
+      if (w == 0 || sw < best) {
+        best = sw;
+        bw = w;
+      }
+    }
+    best_v = a.Z[pix * a.Nw + bw];
+  }
+  if (a.img) a.img[pix] = best_v;
+  if (a.win) a.win[pix] = bw;
+}
+
 }  // namespace
 
 hipError_t launch_lca(const LcaLaunch& a, hipStream_t s) {
   const long npx = (long)a.Nx * a.Ny;
   if (npx <= 0) return hipSuccess;
+  if (a.branch != 0) {
+    const long blocks = (npx + 255) / 256;
+    if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+    if (a.branch == 1) {
+      hipLaunchKernelGGL(lca_ref_argmin_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    } else if (a.branch == 2) {
+      if (a.Nw > 64 * LCA_REF_LANES_W || a.rows_per_chunk < 1 || a.Ny <= a.wy) return hipErrorInvalidValue;
+      const size_t lds = (size_t)a.rows_per_chunk * a.Nw * sizeof(double);
+      if (lds > 64 * 1024) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(lca_ref_sliding_kernel, dim3(a.Nx), dim3(64), lds, s, a);
+    } else if (a.branch == 3) {
+      hipLaunchKernelGGL(lca_ref_general_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    } else {
+      return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   if (a.Nw <= LCA_NW_MAX && a.wy / 2 <= LCA_YM_MAX && a.Nx <= 65535) return launch_tile_nw<1>(a, s);
   const long blocks = (npx + 255) / 256;
   if (blocks > 0x7fffffffL) return hipErrorInvalidValue;

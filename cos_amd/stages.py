@@ -80,8 +80,11 @@ def tdib(mfdata, mtaxe, x_r, D, c, fc, xarr, yarr, *, fill=float("nan"), stream=
     return image
 
 
-def lca(Z, wrf, *, stream=None):
+def lca(Z, wrf, *, mode="reference", stream=None):
     """Window selection: Z [Nx][Ny][Nw] complex, wrf [wy][wx] weights (odd sizes).
+    mode "reference": lcacx.pyx's own output (its branches, its row-Y_MID gap and swapped
+    neighbourhood, FP64 sums in its order; refuses the shapes where it reads or writes
+    outside its arrays); "centred": the intended neighbourhood at every interior pixel.
     Returns (img [Nx][Ny] complex64, win [Nx][Ny] int32) CUDA tensors."""
     import torch
     z = _dev(Z, torch.complex64)
@@ -90,9 +93,11 @@ def lca(Z, wrf, *, stream=None):
         w = w.reshape(1, -1).contiguous()
     Nx, Ny, Nw = z.shape
     wy, wx = w.shape
+    m = {"reference": 0, "centred": 1}[mode]
     img = torch.empty((Nx, Ny), dtype=torch.complex64, device="cuda")
     win = torch.empty((Nx, Ny), dtype=torch.int32, device="cuda")
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
-    check(lib().cos_lca_run(z.data_ptr(), Nx, Ny, Nw, w.data_ptr(), wy, wx, img.data_ptr(), win.data_ptr(), stream))
+    check(lib().cos_lca_run(z.data_ptr(), Nx, Ny, Nw, w.data_ptr(), wy, wx, m, img.data_ptr(), win.data_ptr(),
+                            stream))
     return img, win

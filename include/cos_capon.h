@@ -214,10 +214,23 @@ int cos_tdib_run(const cos_cf32* mfdata, int n_h, int n_t, double t0, double dt,
 /* Window selection: low-complexity adaptive beamformer, replacing
  * framework/beamformer/lca/lcacx.pyx:27-217 (lca(Z, wrf)).  Z [Nx][Ny][Nw] images formed
  * with Nw windows; wrf [wy][wx] neighbourhood weights (odd sizes), device pointers.
- * img [Nx][Ny] gets Z at the window whose weighted neighbourhood power is lowest (first
- * minimum), win [Nx][Ny] that window's index; pixels whose neighbourhood leaves the image
- * get 0 in both.  Either output may be NULL. */
-int cos_lca_run(const cos_cf32* Z, int Nx, int Ny, int Nw, const float* wrf, int wy, int wx,
+ * img [Nx][Ny] gets Z at the selected window (the first minimum of the weighted
+ * neighbourhood power), win [Nx][Ny] its index; either output may be NULL.
+ * mode COS_LCA_REFERENCE: the reference's output index for index, in its own branches
+ *   (chosen from the filter's values, which are read to the host: the call synchronises
+ *   the stream): a 1 x 1 filter with a nonzero weight selects argmin |Z| at every pixel;
+ *   an all-ones range filter (wx = 1) selects by FP64 sliding sums at rows
+ *   Ym+1 .. Ny-Ym-1, writes window 0 at row wy when that row is past them, 0 elsewhere
+ *   (row Ym included); any other filter reads the |Z|^2 array with x and y swapped
+ *   (lcacx.pyx:206; the transposed neighbourhood on square images) and writes the
+ *   interior.  Shapes where the reference reads or writes outside its arrays are refused:
+ *   Ny <= wy for the all-ones filter, Ny > Nx for the others.  Nw <= 256 for the all-ones
+ *   filter.
+ * mode COS_LCA_CENTRED: the intended neighbourhood sum_{yy,xx} wrf[yy][xx]
+ *   |Z[x+xx-Xm][y+yy-Ym][w]|^2 (FP32) at every pixel whose neighbourhood lies inside the
+ *   image, 0 and window 0 elsewhere; any shape. */
+enum { COS_LCA_REFERENCE = 0, COS_LCA_CENTRED = 1 };
+int cos_lca_run(const cos_cf32* Z, int Nx, int Ny, int Nw, const float* wrf, int wy, int wx, int mode,
                 cos_cf32* img, int* win, void* stream);
 
 /* Number of pixels the last cos_capon_run sent to the FP64 fallback (synchronises the

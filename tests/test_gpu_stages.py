@@ -82,7 +82,7 @@ def test_lca_matches_oracle(wy, wx, Nw):
     Nx, Ny = 24, 400
     Z = (rng.standard_normal((Nx, Ny, Nw)) + 1j * rng.standard_normal((Nx, Ny, Nw))).astype(np.complex64)
     wrf = rng.uniform(0.5, 1.0, (wy, wx)).astype(np.float32)
-    img, win = stages.lca(Z, wrf)
+    img, win = stages.lca(Z, wrf, mode="centred")
     img = img.cpu().numpy()
     win = win.cpu().numpy()
     rimg, rwin, s = S.lca(Z.astype(np.complex128), wrf.astype(np.float64))
@@ -108,8 +108,72 @@ def test_lca_weights_as_device_vector():
     rng = np.random.default_rng(5)
     Z = (rng.standard_normal((12, 300, 5)) + 1j * rng.standard_normal((12, 300, 5))).astype(np.complex64)
     w = torch.tensor([0.5, 1.0, 0.5], device="cuda")
-    img, win = stages.lca(torch.from_numpy(Z).cuda(), w)
+    img, win = stages.lca(torch.from_numpy(Z).cuda(), w, mode="centred")
     _, rwin, s = S.lca(Z.astype(np.complex128), np.array([[0.5, 1.0, 0.5]]))
     srt = np.sort(s, axis=-1)
     clear = (srt[..., 1] - srt[..., 0]) > 1e-5 * srt[..., 1]
     np.testing.assert_array_equal(win.cpu().numpy()[1:-1, :][clear], rwin[1:-1, :][clear])
+
+
+# ---------------------------------------------------------------- lca, the reference's own output
+
+def _lca_ref_case(Z, wrf):
+    from cos_amd import stages
+    img, win = stages.lca(Z, wrf, mode="reference")
+    rimg, rwin = S.lca_reference(Z.astype(np.complex128), np.asarray(wrf, np.float64))
+    np.testing.assert_array_equal(win.cpu().numpy(), rwin)
+    np.testing.assert_array_equal(img.cpu().numpy(), rimg.astype(np.complex64))
+
+
+@pytest.mark.parametrize("Nx,Ny,Nw,wrf", [
+    (24, 400, 6, [[0.5]]),                          # 1 x 1: argmin |Z| everywhere
+    (24, 400, 6, [[1.0]] * 3),                      # all-ones range filters: sliding FP64 sums
+    (24, 400, 20, [[1.0]] * 5),
+    (8, 300, 70, [[1.0]] * 9),                      # two windows per lane
+    (4, 200, 4, [[1.0]] * 67),                      # long filter, chunks of 64 sliding rows
+    (6, 7, 5, [[1.0]] * 5),                         # row Y_AVG keeps the first segment's window 0
+    (64, 64, 6, np.arange(1, 10, dtype=np.float32).reshape(3, 3)),   # general, square: swapped
+    (48, 48, 5, np.full((3, 5), 0.5, np.float32)),
+    (40, 30, 6, np.array([[1.0, 2.0, 1.0], [2.0, 4.0, 2.0], [1.0, 2.0, 1.0]], np.float32)),  # Ny < Nx
+    (30, 20, 4, [[1.0, 1.0, 1.0]]),                 # all ones but across beams: the general branch
+    (20, 16, 3, [[0.0]]),                           # a zero 1 x 1 weight: the general branch
+])
+def test_lca_reference_mode_index_exact(Nx, Ny, Nw, wrf):
+    """Reference mode against the oracle's restatement of lcacx.pyx: windows and values
+    equal at every pixel (the FP64 sums follow the reference's order and rounding)."""
+    say("lca reference", Nx, Ny, Nw, np.shape(wrf))
+    rng = np.random.default_rng(Nx * 7 + Ny + Nw)
+    Z = (rng.standard_normal((Nx, Ny, Nw)) + 1j * rng.standard_normal((Nx, Ny, Nw))).astype(np.complex64)
+    _lca_ref_case(Z, np.asarray(wrf, np.float32))
+
+
+def test_lca_reference_mode_summation_order():
+    """The hand-worked cases where the reference's summation order decides the window
+    (2^53 absorbing the 1s that follow): the GPU chooses what the reference's order gives,
+    not what exact sums give."""
+    say("lca reference rounding")
+    Z = np.zeros((1, 8, 2), dtype=np.complex64)
+    Z[0, 0, 0] = 2.0 ** 26 + 1j * 2.0 ** 26
+    Z[0, 1:, 0] = 1.0
+    Z[0, 1:3, 1] = 1.0
+    _lca_ref_case(Z, np.ones((3, 1), np.float32))
+    from cos_amd import stages
+    _, win = stages.lca(Z, np.ones((3, 1), np.float32), mode="reference")
+    assert win.cpu().numpy()[0, 2] == 0
+    Z = np.zeros((3, 3, 2), dtype=np.complex64)
+    Z[1, :, 0] = [2.0 ** 26 + 1j * 2.0 ** 26, 1, 1]
+    Z[1, :, 1] = [2.0 ** 26 + 1j * 2.0 ** 26, 0, 1]
+    _lca_ref_case(Z, np.ones((1, 3), np.float32))
+    _, win = stages.lca(Z, np.ones((1, 3), np.float32), mode="reference")
+    assert win.cpu().numpy()[1, 1] == 0
+
+
+def test_lca_reference_mode_refuses_out_of_array_shapes():
+    from cos_amd import stages
+    from cos_amd._lib import CosError
+    say("lca reference refusals")
+    Z = np.zeros((4, 5, 3), np.complex64)
+    with pytest.raises(CosError, match="Ny > its length"):
+        stages.lca(Z, np.ones((5, 1), np.float32), mode="reference")
+    with pytest.raises(CosError, match="Ny <= Nx"):
+        stages.lca(Z, np.ones((3, 3), np.float32), mode="reference")
