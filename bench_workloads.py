@@ -97,7 +97,7 @@ def aperture(args):
                  npx / (ms * 1e-3), "pixels/s", ms,
                  {"workload": "expanding-aperture frame", "Nx": Nx, "Nz": Nz, "Mmax": Mmax, "Yavg": Y,
                   "pixels": npx, "ill_rows_d": 1e-5, "fallback_pixels": nfb})
-    line["roofline"] = {"bound": "mfma", "achieved": round(fpx * npx / (ms * 1e-3) / 1e12, 3),
+    line["roofline"] = {"bound": "valu", "achieved": round(fpx * npx / (ms * 1e-3) / 1e12, 3),
                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(fpx * npx / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
                         "flop_per_pixel_mean": round(fpx)}

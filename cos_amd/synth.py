@@ -79,17 +79,25 @@ def speckle_cube(Nx: int, Nz: int, M: int, seed: int, n_scatterers: int = 10,
         cube += h[i] * n[:, i:i + Nz, :]
     del n
     var = float(np.sum(h * h))
-    u = uniforms(seed, stream + 1, 3 * n_scatterers)
     amp = np.sqrt(10.0 ** (scatterer_db / 10.0) * var)
-    for s in range(n_scatterers):
-        zs = min(int(u[3 * s] * Nz), Nz - 1)
-        xs = min(int(u[3 * s + 1] * Nx), Nx - 1)
-        a = amp * np.exp(2j * np.pi * u[3 * s + 2])
+    for zs, xs, ph in _scatterers(Nx, Nz, seed, n_scatterers, stream):
+        a = amp * np.exp(2j * np.pi * ph)
         for i in range(5):
             zz = zs + i - 2
             if 0 <= zz < Nz:
                 cube[xs, zz, :] += h[i] * a
     return cube.astype(np.complex64)
+
+
+def _scatterers(Nx, Nz, seed, n_scatterers, stream):
+    u = uniforms(seed, stream + 1, 3 * n_scatterers)
+    return [(min(int(u[3 * s] * Nz), Nz - 1), min(int(u[3 * s + 1] * Nx), Nx - 1), u[3 * s + 2])
+            for s in range(n_scatterers)]
+
+
+def scatterer_positions(Nx: int, Nz: int, seed: int, n_scatterers: int = 10, stream: int = 0):
+    """(range row z, beam x) of each point scatterer speckle_cube() places."""
+    return [(z, x) for z, x, _ in _scatterers(Nx, Nz, seed, n_scatterers, stream)]
 
 
 def config2(Nx: int = 256, Nz: int = 2048, M: int = 64, seed: int = 2) -> np.ndarray:
@@ -245,12 +253,9 @@ def speckle_cube_torch(Nx: int, Nz: int, M: int, seed: int, device="cuda", n_sca
         cube += float(h[i]) * n[:, i:i + Nz, :]
     del n
     var = float(np.sum(h * h))
-    u = uniforms(seed, stream + 1, 3 * n_scatterers)
     amp = np.sqrt(10.0 ** (scatterer_db / 10.0) * var)
-    for s in range(n_scatterers):
-        zs = min(int(u[3 * s] * Nz), Nz - 1)
-        xs = min(int(u[3 * s + 1] * Nx), Nx - 1)
-        a = amp * np.exp(2j * np.pi * u[3 * s + 2])
+    for zs, xs, ph in _scatterers(Nx, Nz, seed, n_scatterers, stream):
+        a = amp * np.exp(2j * np.pi * ph)
         for i in range(5):
             zz = zs + i - 2
             if 0 <= zz < Nz:
