@@ -34,8 +34,12 @@ int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
+// The error is reported here and cleared from HIP's per-thread last error, which a
+// failed runtime call (an allocation past the device's memory) leaves set: the next
+// launch's hipGetLastError would otherwise report it again.
 int fail_hip(hipError_t e, const char* where) {
   g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+  (void)hipGetLastError();
   return (int)e;
 }
 
@@ -53,6 +57,7 @@ struct DevBuf {
     n = 0;
     hipError_t e = hipMalloc(&p, bytes);
     if (e == hipSuccess) n = bytes;
+    else p = nullptr;
     return e;
   }
   void release() {
@@ -285,8 +290,6 @@ int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
   if (R && tables) return fail(hipErrorInvalidValue, "R output needs a uniform L");
   if (w && wstride < Lmax) return fail(hipErrorInvalidValue, "wstride < L");
   if (ds.nsteer < 0 || (ds.nsteer > 0 && !ds.steer_sin)) return fail(hipErrorInvalidValue, "bad steering");
-  if (ds.precision == COS_PREC_FP32_FALLBACK && ds.nsteer > 0)
-    return fail(hipErrorInvalidValue, "FP64 fallback does not support steering");
   if (ds.z_mode != COS_Z_AMPLITUDE && ds.z_mode != COS_Z_POWER) return fail(hipErrorInvalidValue, "bad z_mode");
   if (ds.p_mode != COS_P_LOADED && ds.p_mode != COS_P_UNLOADED) return fail(hipErrorInvalidValue, "bad p_mode");
   if (ds.nsteer > 0 && (ds.z_mode != COS_Z_AMPLITUDE || ds.p_mode != COS_P_LOADED))

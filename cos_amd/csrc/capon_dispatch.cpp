@@ -41,10 +41,6 @@ hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_
 }
 
 static hipError_t f64_launch(const CaponLaunch& a, int Lmax, int Mmax, bool list, hipStream_t s, const char** why) {
-  if (a.nsteer > 0) {
-    if (why) *why = "steering vectors are not supported on the FP64 path";
-    return hipErrorInvalidValue;
-  }
   if (Lmax <= 16)
     return list ? launch_one<double, 16, 1, 0, true, true, false>(a, Mmax, s) : launch_one<double, 16, 1, 0, true, false, false>(a, Mmax, s);
   if (Lmax <= 32)

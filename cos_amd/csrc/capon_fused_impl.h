@@ -85,8 +85,8 @@ __device__ __noinline__ cplx<T> steered_mean(const float2* xc, int c, int K, int
   cplx<T> acc = {T(0), T(0)};
   if (c < L) {
     for (int k = 0; k < K; ++k) {
-      float s2, c2;
-      sincospif(-(float)(sn * (T)k), &s2, &c2);
+      T s2, c2;
+      sincospi_t<T>(-(sn * (T)k), &s2, &c2);
       const float2 xv = xc[k + c];
       add_mul(acc, cplx<T>{(T)xv.x, (T)xv.y}, cplx<T>{(T)c2, (T)s2});
     }
@@ -633,8 +633,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       static_for<0, CPL>([&](auto S_) {
         constexpr int S = decltype(S_)::value;
         const int cc = cs[S];
-        float sv, cv;
-        sincospif((float)(sn * (T)cc), &sv, &cv);
+        T sv, cv;
+        sincospi_t<T>(sn * (T)cc, &sv, &cv);
         bs[S] = cc < L ? cplx<T>{(T)cv, (T)sv} : cplx<T>{T(0), T(0)};
         gcur[S] = steered_mean<T>(xcg, cc, K, L, sn, invK);
       });
@@ -787,8 +787,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           constexpr int S = decltype(S_)::value;
           av[S] = {T(1), T(0)};
           if (steered) {
-            float sv, cv;
-            sincospif((float)(sn * (T)cs[S]), &sv, &cv);
+            T sv, cv;
+            sincospi_t<T>(sn * (T)cs[S], &sv, &cv);
             av[S] = {(T)cv, (T)sv};
           }
         });
@@ -825,8 +825,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         cplx<T> wv = {y[S].x * sc, y[S].y * sc};
         if (cc < L) add_conj_mul(zp, wv, gcur[S]);
         if (steered) {
-          float sv, cv;
-          sincospif(-(float)(sn * (T)cc), &sv, &cv);
+          T sv, cv;
+          sincospi_t<T>(-(sn * (T)cc), &sv, &cv);
           cplx<T> r = {T(0), T(0)};
           add_mul(r, wv, cplx<T>{(T)cv, (T)sv});
           wv = r;

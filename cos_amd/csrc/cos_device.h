@@ -211,6 +211,12 @@ __device__ __forceinline__ void bcast_sub_conj_mul2(cplx<T>& a, cplx<T>& b, cplx
   }
 }
 
+// sin(pi x) and cos(pi x) in the precision of T (the FP64 kernels take steering phases in
+// double, so their power and weights carry no FP32 phase rounding)
+template <typename T> __device__ __forceinline__ void sincospi_t(T x, T* s, T* c);
+template <> __device__ __forceinline__ void sincospi_t<float>(float x, float* s, float* c) { sincospif(x, s, c); }
+template <> __device__ __forceinline__ void sincospi_t<double>(double x, double* s, double* c) { sincospi(x, s, c); }
+
 template <typename T> __device__ __forceinline__ T rsqrt_t(T v);
 // v_rsq_f32 (1 ulp); the FP32 path's accuracy budget is set by the covariance sums.
 template <> __device__ __forceinline__ float rsqrt_t<float>(float v) { return __builtin_amdgcn_rsqf(v); }

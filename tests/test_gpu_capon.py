@@ -550,6 +550,50 @@ def test_steering_full_bins_refined_within_1e4():
     assert np.abs(z1 - ref["z"]).max() <= 1e-4 * np.abs(ref["z"]).max()
 
 
+@pytest.mark.parametrize("L", [16, 40])
+def test_steering_with_fp64_fallback(L):
+    """Steered frames on the FP64 paths: per-row loading 1e-6 on every third range bin
+    (condition bound 1 + L/d past 1e6) sends those bins to the FP64 kernel, which solves
+    every angle of the flagged pixel; all powers, weights and amplitudes match the oracle
+    (refined FP32 elsewhere), and the flagged bins match an all-FP64 steered run bit for
+    bit (16 lanes and 64 lanes in FP64)."""
+    import torch
+    from cos_amd import synth
+    M, Y = 2 * L, 2
+    say("steering + fallback", 1, 256, M, L, 12)
+    x = synth.config1(N=256, M=M, seed=1)
+    sines = synth.steering_sines(12).astype(np.float32)
+    st = torch.from_numpy(sines).cuda()
+    Nz = x.shape[1]
+    d_rows = np.full(Nz, 0.01, np.float32)
+    d_rows[::3] = 1e-6
+    rows = {"L": torch.full((Nz,), L, dtype=torch.int32).cuda(), "M": torch.full((Nz,), M, dtype=torch.int32).cuda(),
+            "off": torch.zeros(Nz, dtype=torch.int32).cuda(), "d": torch.from_numpy(d_rows).cuda()}
+    # the FP32 pixels take one refinement step (unrefined steered FP32 reaches ~1.3e-4 next
+    # to the 20 dB sources); the flagged ones go to the FP64 kernel
+    o, plan = _plan_run(x, 0.0, 0, Y, rows=rows, steer_sin=st, precision="fp32+fallback", refine=True,
+                        want=("P", "z", "w", "flags"))
+    o64, _ = _plan_run(x, 0.0, 0, Y, rows=rows, steer_sin=st, precision="fp64")
+    S = len(sines)
+    flagged = np.nonzero(o["flags"])[0]
+    c = np.arange(Nz - 2 * Y) + Y
+    assert plan.fallback_count() == len(flagged) == int((d_rows[c] < 1e-3).sum())
+    P, P64 = o["P"].reshape(-1, S), o64["P"].reshape(-1, S)
+    np.testing.assert_array_equal(P[flagged], P64[flagged])     # the same FP64 kernel
+    np.testing.assert_array_equal(o["w"].reshape(-1, S, L)[flagged], o64["w"].reshape(-1, S, L)[flagged])
+    for dval in (0.01, 1e-6):
+        sel = np.nonzero(d_rows[c] == np.float32(dval))[0]
+        ref = O.capon_steered(x.astype(np.complex128), float(np.float32(dval)), L, Y, sines.astype(np.float64),
+                              pixels=(np.zeros(len(sel), int), sel))
+        tolP = 1e-4 if dval > 1e-3 else 1e-6
+        assert rel(P[sel], ref["P"]).max() <= tolP, dval
+        assert rel(P64[sel], ref["P"]).max() <= 1e-6, dval
+        w = o["w"].reshape(-1, S, L)[sel]
+        assert np.abs(w - ref["w"]).max() <= 1e-3 * np.abs(ref["w"]).max(), dval
+        z = o["z"].reshape(-1, S)[sel]
+        assert np.abs(z - ref["z"]).max() <= 1e-4 * np.abs(ref["z"]).max(), dval
+
+
 # ------------------------------------------------------------------ beamspace drop-in (Nb > 0)
 
 @pytest.mark.parametrize("M,L,Nb", [(12, 6, 2), (32, 16, 8), (32, 16, 9), (64, 32, 5)])
@@ -617,6 +661,94 @@ def test_solver_component(N):
     np.testing.assert_allclose(x, ref, rtol=0, atol=1e-4 * np.abs(ref).max())
     x1 = solveGaussJordanGPU(A[0], b[0])
     np.testing.assert_allclose(x1[0], ref[0], rtol=0, atol=1e-4 * np.abs(ref[0]).max())
+
+
+@pytest.mark.parametrize("R_on_gpu,x_on_gpu", [(True, True), (True, False), (False, True)])
+def test_buildr_device_pointers(R_on_gpu, x_on_gpu):
+    """BuildR::getR with device buffers (RTC/BuildR.cpp:59-103): R and/or x as device
+    addresses of torch tensors give the host call's R, element space and beamspace input."""
+    import torch
+    from cos_amd import BuildR
+    say("buildR on gpu", R_on_gpu, x_on_gpu)
+    x = uniform_cube(6, 3, 12, 24)
+    Nx, Nz, M = x.shape
+    L, Y, d = 9, 1, 0.2
+    shape = (Nx, Nz - 2 * Y, L, L)
+    xd = torch.from_numpy(x).cuda()
+    Rd = torch.full(shape, float("nan"), dtype=torch.complex64, device="cuda")
+    Rh = np.zeros(shape, np.complex64)
+    R_arg = Rd.data_ptr() if R_on_gpu else Rh
+    x_arg = xd.data_ptr() if x_on_gpu else x
+    assert BuildR().getR(R_arg, x_arg, d, L, Y, M, Nx, Nz, 0, R_on_gpu=R_on_gpu, x_on_gpu=x_on_gpu) == 0
+    R = Rd.cpu().numpy() if R_on_gpu else Rh
+    ref = O.reference_gpu_outputs(x.astype(np.complex128), float(np.float32(d)), L, Y)["R"]
+    assert np.abs(R - ref).max() <= 1e-5 * np.abs(ref).max()
+    # beamspace-transformed input [Nx][Nz][K][Nb] (buildR_full, loading d tr/Nb)
+    rng = np.random.default_rng(10)
+    K, Nb = M - L + 1, 4
+    xbs = (rng.standard_normal((Nx, Nz, K, Nb)) + 1j * rng.standard_normal((Nx, Nz, K, Nb))).astype(np.complex64)
+    Rb = torch.empty((Nx, Nz - 2 * Y, Nb, Nb), dtype=torch.complex64, device="cuda")
+    Rbh = np.zeros((Nx, Nz - 2 * Y, Nb, Nb), np.complex64)
+    xb_arg = torch.from_numpy(xbs).cuda() if x_on_gpu else xbs
+    assert BuildR().getR(Rb.data_ptr() if R_on_gpu else Rbh, xb_arg.data_ptr() if x_on_gpu else xb_arg, d, L, Y,
+                         M, Nx, Nz, Nb, R_on_gpu=R_on_gpu, x_on_gpu=x_on_gpu) == 0
+    Rb = Rb.cpu().numpy() if R_on_gpu else Rbh
+    Rbh2 = np.zeros_like(Rbh)
+    assert BuildR().getR(Rbh2, xbs, d, L, Y, M, Nx, Nz, Nb) == 0
+    np.testing.assert_array_equal(Rb, Rbh2)
+    for ix, j in [(0, 0), (2, Nz - 2 * Y - 1)]:
+        v = xbs[ix, j:j + 2 * Y + 1].reshape(-1, Nb).astype(complex)
+        S = v.T @ v.conj()
+        S += d * np.trace(S).real / Nb * np.eye(Nb)
+        np.testing.assert_allclose(Rb[ix, j], np.conj(S), rtol=0, atol=1e-5 * np.abs(S).max())
+
+
+@pytest.mark.parametrize("flags", [(1, 1, 1), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (0, 1, 1), (1, 0, 1)])
+def test_solver_device_pointers(flags):
+    """Solver::solve with device buffers (RTC/Solver.cpp:59-113) for every combination of
+    x_on_gpu, A_on_gpu, b_on_gpu, against LAPACK on the column-major systems."""
+    import torch
+    from cos_amd import Solver
+    xg, Ag, bg = (bool(f) for f in flags)
+    say("solver on gpu", flags)
+    rng = np.random.default_rng(sum(flags) * 3 + flags[0])
+    N, batch = 40, 7
+    A = (rng.standard_normal((batch, N, N)) + 1j * rng.standard_normal((batch, N, N)) + 12 * np.eye(N)).astype(np.complex64)
+    b = (rng.standard_normal((batch, N)) + 1j * rng.standard_normal((batch, N))).astype(np.complex64)
+    Ad, bd = torch.from_numpy(A).cuda(), torch.from_numpy(b).cuda()
+    xd = torch.full((batch, N), float("nan"), dtype=torch.complex64, device="cuda")
+    xh = np.zeros((batch, N), np.complex64)
+    assert Solver().solve(xd.data_ptr() if xg else xh, Ad.data_ptr() if Ag else A, bd.data_ptr() if bg else b, N,
+                          batch, x_on_gpu=xg, A_on_gpu=Ag, b_on_gpu=bg) == 0
+    x = xd.cpu().numpy() if xg else xh
+    ref = np.linalg.solve(np.swapaxes(A, 1, 2).astype(complex), b.astype(complex)[..., None])[..., 0]
+    np.testing.assert_allclose(x, ref, rtol=0, atol=1e-4 * np.abs(ref).max())
+
+
+def test_out_of_memory_is_reported_and_the_plan_survives():
+    """A request past the device's memory (RTC/Capon.cpp:116-133 reports
+    cudaErrorMemoryValueTooLarge) returns the out-of-memory code with a message, and the
+    same process keeps working afterwards."""
+    import torch
+    from cos_amd import getCaponCUDAPy
+    from cos_amd._lib import lib
+    say("out of memory")
+    free, total = torch.cuda.mem_get_info()
+    # R of 6000 x 4092 pixels at L = 64 is 804 GB, past the device; the cube itself (12.6 GB)
+    # is allocated first, then R fails before anything is copied from the host pointers
+    M, L, Y, Nx, Nz = 64, 64, 2, 6000, 4096
+    need_R = Nx * (Nz - 2 * Y) * L * L * 8
+    assert need_R > 2 * total
+    h = np.zeros(1, np.complex64)
+    code = lib().cos_capon_get_capon_p(None, None, h.ctypes.data, None, h.ctypes.data, 0.01, L, Y, M, Nx, Nz, 0)
+    assert code == 2, code                      # hipErrorOutOfMemory
+    assert b"device buffer" in lib().cos_capon_last_error()
+    # the drop-in still works afterwards
+    xs = uniform_cube(7, 2, 12, 16)
+    z2, R2, w2, e = getCaponCUDAPy(xs, 0.1, 8, 1, np.array([0]), False, False)
+    assert e == 0
+    ref = O.reference_gpu_outputs(xs.astype(np.complex128), float(np.float32(0.1)), 8, 1)
+    assert np.abs(z2 - ref["z"]).max() <= 2e-4 * np.abs(ref["z"]).max() + 1e-6
 
 
 # ------------------------------------------------------------------ edge cases
