@@ -70,11 +70,6 @@ struct DevBuf {
 
 }  // namespace
 
-// Output rows jlo .. jlo+jn-1 whose centre-row L falls in one lane-shape bucket.
-struct RowRun {
-  int jlo, jn, Lmax, shape;
-};
-
 // Autotuned lane shapes: for each device and each frame configuration that changes the
 // kernels' cost (largest aperture, window, options), the fastest shape per subarray
 // length L = 1..64 as measured by cos_capon_autotune (0 = not tuned: default_shape,
@@ -137,21 +132,16 @@ struct cos_capon_plan {
   hipStream_t last_stream = nullptr;
   hipStream_t capture_stream = nullptr;  // graphs are captured here, launched on the caller's
   FrameGraph fg;
-  std::vector<RowRun> runs;   // lane-shape runs of output rows for table-driven frames
-  TuneTable shapes{};         // the tuned shapes the current frame uses
+  DevBuf rows;                // table-driven frames: per-bucket counts [8], then row lists [nb][Nzp]
   bool fallback_ran = false;
-  // pixels the condition bound flags in the current frame (host count from the L and d
-  // tables): the FP64 fallback runs one block per group over this many list entries plus
-  // a slack, and a bounded persistent grid over any entries past them
-  long fb_bound = 0;
   unsigned buf_gen = 0;       // bumped by every reallocation of a plan buffer
   cos_capon_plan() {
-    for (DevBuf* b : {&flags, &list, &count, &x, &z, &w, &R, &P, &B, &bs_tw}) b->gen = &buf_gen;
+    for (DevBuf* b : {&flags, &list, &count, &x, &z, &w, &R, &P, &B, &bs_tw, &rows}) b->gen = &buf_gen;
   }
   ~cos_capon_plan() {
     fg.reset();
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
-    for (DevBuf* b : {&flags, &list, &count, &x, &z, &w, &R, &P, &B, &bs_tw}) b->release();
+    for (DevBuf* b : {&flags, &list, &count, &x, &z, &w, &R, &P, &B, &bs_tw, &rows}) b->release();
   }
 };
 
@@ -248,14 +238,46 @@ int cos_capon_plan_destroy(cos_capon_plan* plan) {
 
 namespace {
 
+// Launch buckets of a table-driven frame: every subarray length 1 .. Lmax that a row may
+// hold is given a bucket (its FP32 lane shape, or its FP64 kernel), one launch per bucket.
+// Which rows fall in which bucket is decided on the device at every run of the frame
+// (cosk::plan_rows, captured in the frame graph), so the host never reads the tables and
+// a replay follows their current contents.
+struct Buckets {
+  cosk::RowBuckets bk{};
+  int nb = 0;
+  int cap[8] = {};     // largest L of the bucket's rows (its launch's Lcap)
+  int shape[8] = {};   // FP32: lane shape
+};
+
+Buckets make_buckets(const cos_capon_desc& ds, const TuneTable& shapes) {
+  Buckets r;
+  int of[65];
+  for (int L = 1; L <= ds.Lmax; ++L) {
+    const int key = ds.precision == COS_PREC_FP64 ? (L <= 16 ? 1 : L <= 32 ? 2 : 3) : shape_for(shapes, L);
+    int b = 0;
+    while (b < r.nb && r.shape[b] != key) ++b;
+    if (b == r.nb) {
+      r.shape[b] = key;
+      ++r.nb;
+    }
+    r.cap[b] = L;
+    of[L] = b;
+  }
+  // a length outside 1 .. Lmax goes to the bucket of Lmax, whose launch writes that row as
+  // NaN with flag 4 (the kernels' validity test)
+  for (int L = 0; L <= 64; ++L) r.bk.b[L] = (unsigned char)(L >= 1 && L <= ds.Lmax ? of[L] : of[ds.Lmax]);
+  return r;
+}
+
 // What one frame's launches need beyond the descriptor and the caller's buffers: the
-// lane-shape runs of a table-driven frame, the tuned shapes, the host's count of pixels
-// the condition bound flags, and device scratch (per-pixel flags when the caller passes
-// none, the compacted FP64 pixel list and its device-side length, the matrix-core
-// covariance when the caller passes no R).
+// buckets and device row lists of a table-driven frame, the tuned shapes, the count of
+// pixels the condition bound flags (known to the host for uniform frames), and device
+// scratch (per-pixel flags when the caller passes none, the compacted FP64 pixel list and
+// its device-side length, the matrix-core covariance when the caller passes no R).
 struct FrameCtx {
-  const RowRun* runs = nullptr;
-  int nruns = 0;
+  Buckets bu;
+  int* rows = nullptr;   // [8] counts, then [nb][Nzp] row lists
   TuneTable shapes{};
   long fb_bound = 0;
   uint8_t* flags = nullptr;
@@ -263,13 +285,6 @@ struct FrameCtx {
   int* count = nullptr;
   float2* R = nullptr;
 };
-
-struct RunSpan {
-  const RowRun *b, *e;
-  const RowRun* begin() const { return b; }
-  const RowRun* end() const { return e; }
-};
-inline RunSpan ctx_runs(const FrameCtx& c) { return {c.runs, c.runs + c.nruns}; }
 
 // Enqueue one frame on stream s (arguments already validated; scratch already sized).
 int enqueue_frame(const FrameCtx& c, const cos_capon_desc& ds, const cos_cf32* x, float* P, cos_cf32* z,
@@ -330,19 +345,27 @@ int enqueue_frame(const FrameCtx& ctx, const cos_capon_desc& ds, const cos_cf32*
   a.refine = ds.refine ? 1 : 0;
   const char* why = nullptr;
   hipError_t e;
+  const int Nzp = ds.Nz - 2 * ds.Yavg;
+  if (tables && (e = cosk::plan_rows(ds.row_L, Nzp, ds.Yavg, ctx.bu.bk, ctx.bu.nb, ctx.rows + 8, ctx.rows, s)) !=
+                    hipSuccess)
+    return fail_hip(e, "row planning");
+  // bucket b's launch: its device row list, a grid for every row (blocks past the count exit)
+  auto bucket_launch = [&](cosk::CaponLaunch& c, int b) {
+    c.rlist = ctx.rows + 8 + (size_t)b * Nzp;
+    c.rcount = ctx.rows + b;
+    c.npix = ds.Nx * Nzp;
+    c.Lcap = ctx.bu.cap[b];
+  };
   if (ds.cov_mfma) {
     // covariance on the matrix cores into R (the caller's, or the plan's; Lmax^2 per pixel),
     // then the solve reads it back; one launch per row run, with one or four 32 x 32 tiles
     cosk::CaponLaunch c = a;
     c.R = R ? (float2*)R : ctx.R;
     const int Rs = Lmax * Lmax;
-    if (tables && ctx.nruns) {
-      for (const RowRun& r : ctx_runs(ctx)) {
-        c.jlo = r.jlo;
-        c.jn = r.jn;
-        c.npix = ds.Nx * r.jn;
-        c.Lcap = r.Lmax;   // every row of the run holds its L within the run's shape
-        if ((e = cosk::launch_cov_mfma(c, r.Lmax > 32 ? 2 : 1, Mmax, Rs, s)) != hipSuccess)
+    if (tables) {
+      for (int b = 0; b < ctx.bu.nb; ++b) {
+        bucket_launch(c, b);
+        if ((e = cosk::launch_cov_mfma(c, ctx.bu.cap[b] > 32 ? 2 : 1, Mmax, Rs, s)) != hipSuccess)
           return fail_hip(e, "MFMA covariance");
       }
     } else if ((e = cosk::launch_cov_mfma(c, Lmax > 32 ? 2 : 1, Mmax, Rs, s)) != hipSuccess) {
@@ -353,13 +376,11 @@ int enqueue_frame(const FrameCtx& ctx, const cos_capon_desc& ds, const cos_cf32*
     a.Rstride = Rs;
   }
   if (ds.precision == COS_PREC_FP64) {
-    if (tables && ctx.nruns) {
-      for (const RowRun& r : ctx_runs(ctx)) {
-        cosk::CaponLaunch b = a;
-        b.jlo = r.jlo;
-        b.jn = r.jn;
-        b.npix = ds.Nx * r.jn;
-        e = cosk::launch_capon_f64(b, r.Lmax, Mmax, s, &why);
+    if (tables) {
+      for (int b = 0; b < ctx.bu.nb; ++b) {
+        cosk::CaponLaunch c = a;
+        bucket_launch(c, b);
+        e = cosk::launch_capon_f64(c, ctx.bu.cap[b], Mmax, s, &why);
         if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 kernel");
       }
       return 0;
@@ -372,14 +393,12 @@ int enqueue_frame(const FrameCtx& ctx, const cos_capon_desc& ds, const cos_cf32*
   uint8_t* fl = (uint8_t*)flags;
   if (fb && !fl) fl = ctx.flags;
   a.flags = fl;
-  if (tables && ctx.nruns) {
-    // range rows grouped into runs of one L bucket, each launched with its lane shape
-    for (const RowRun& r : ctx_runs(ctx)) {
-      cosk::CaponLaunch b = a;
-      b.jlo = r.jlo;
-      b.jn = r.jn;
-      b.npix = ds.Nx * r.jn;
-      e = cosk::launch_capon_f32_shape(b, r.shape, Mmax, s, &why);
+  if (tables) {
+    // each lane shape over the rows of its bucket
+    for (int b = 0; b < ctx.bu.nb; ++b) {
+      cosk::CaponLaunch c = a;
+      bucket_launch(c, b);
+      e = cosk::launch_capon_f32_shape(c, ctx.bu.shape[b], Mmax, s, &why);
       if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
     }
   } else {
@@ -447,49 +466,21 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   const int gen = g_tune_gen.load();
   const bool replay = ds.graph && g.valid && same_desc(g.ds, ds) && std::memcmp(g.ptr, ptr, sizeof(ptr)) == 0 &&
                       g.wstride == wstride && g.tune_gen == gen && g.buf_gen == plan->buf_gen;
-  plan->runs.clear();
-  plan->shapes = tuned_shapes(ds);
-  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
-  if (!tables) {
-    const float th = (float)(ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6);
-    plan->fb_bound = (1.f + (float)ds.L / ds.d) > th ? npix : 0;
-  }
-  if (tables && !replay) {
-    // Read the L table (Nz ints) to split the rows into runs of one lane shape (the tuned
-    // one, or default_shape); more than 16 runs fall back to a single launch sized for
-    // Lmax.  With the fallback on, the d table too, for the bound's pixel count.  (ordered
-    // after the caller's work on s, which may still be writing the tables)
-    std::vector<int> hL(ds.Nz, ds.L);
-    std::vector<float> hd(ds.Nz, ds.d);
-    if ((ds.row_L &&
-         (e = hipMemcpyAsync(hL.data(), ds.row_L, sizeof(int) * ds.Nz, hipMemcpyDeviceToHost, s)) != hipSuccess) ||
-        (fb && ds.row_d &&
-         (e = hipMemcpyAsync(hd.data(), ds.row_d, sizeof(float) * ds.Nz, hipMemcpyDeviceToHost, s)) != hipSuccess) ||
-        (e = hipStreamSynchronize(s)) != hipSuccess)
-      return fail_hip(e, "read L table");
-    const int Nzp = ds.Nz - 2 * ds.Yavg;
-    if (fb) {   // the kernel's bound test, (1 + L/d) > cond_thresh in FP32, per centre row
-      const float th = (float)(ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6);
-      long rows = 0;
-      for (int jj = 0; jj < Nzp; ++jj)
-        rows += (1.f + (float)hL[jj + ds.Yavg] / hd[jj + ds.Yavg]) > th ? 1 : 0;
-      plan->fb_bound = rows * ds.Nx;
-    }
-    for (int jj = 0; jj < Nzp;) {
-      const int sh = shape_for(plan->shapes, hL[jj + ds.Yavg]);
-      int j2 = jj + 1;
-      while (j2 < Nzp && shape_for(plan->shapes, hL[j2 + ds.Yavg]) == sh) ++j2;
-      const int cap = cosk::shape_cap(sh);
-      plan->runs.push_back({jj, j2 - jj, cap < ds.Lmax ? cap : ds.Lmax, sh});
-      jj = j2;
-    }
-    if (plan->runs.size() > 16) plan->runs.clear();
-  }
   FrameCtx ctx;
-  ctx.runs = plan->runs.data();
-  ctx.nruns = (int)plan->runs.size();
-  ctx.shapes = plan->shapes;
-  ctx.fb_bound = plan->fb_bound;
+  ctx.shapes = tuned_shapes(ds);
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  if (tables) {
+    ctx.bu = make_buckets(ds, ctx.shapes);
+    const size_t Nzp = ds.Nz - 2 * ds.Yavg;
+    if ((e = plan->rows.ensure(sizeof(int) * (8 + Nzp * ctx.bu.nb))) != hipSuccess) return fail_hip(e, "row lists");
+    // the bound's count is not known without the tables: the FP64 head covers a sixteenth
+    // of the pixels plus a slack, the persistent grid the rest
+    ctx.fb_bound = npix / 16;
+  } else {
+    const float th = (float)(ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6);
+    ctx.fb_bound = (1.f + (float)ds.L / ds.d) > th ? npix : 0;
+  }
+  ctx.rows = plan->rows.as<int>();
   ctx.flags = plan->flags.as<uint8_t>();
   ctx.list = plan->list.as<int>();
   ctx.count = plan->count.as<int>();

@@ -120,7 +120,16 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   const int gid = gid0 + q;
   const bool active = gid < npix;
   const int gsel = active ? gid : gid0;
-  const int p = a.plist ? a.plist[gsel] : (gsel / a.jn) * Nzp + a.jlo + gsel % a.jn;
+  int p;
+  if (a.plist) {
+    p = a.plist[gsel];
+  } else if (a.rlist) {
+    const int nr = __builtin_amdgcn_readfirstlane(*a.rcount);   // > 0 when any group is active
+    const int ixr = gsel / nr;
+    p = ixr * Nzp + a.rlist[gsel - ixr * nr];
+  } else {
+    p = (gsel / a.jn) * Nzp + a.jlo + gsel % a.jn;
+  }
   const int ix = p / Nzp, j = p - ix * Nzp, crow = j + Y;
   int L = a.rowL ? a.rowL[crow] : a.L;
   int M = a.rowM ? a.rowM[crow] : a.M;
@@ -1033,7 +1042,7 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   // list (the FP64 fallback) can use a bounded grid; otherwise one block per group (a
   // loop costs the full-frame kernels registers).
   constexpr int NGk = Grp<G>::NG;
-  const int npix = a.pcount ? min(*a.pcount - a.pbase, a.npix) : a.npix;
+  const int npix = a.pcount ? min(*a.pcount - a.pbase, a.npix) : a.rcount ? min(a.Nx * *a.rcount, a.npix) : a.npix;
   if constexpr (PERSIST) {
     for (int gid0 = blockIdx.x * NGk; gid0 < npix; gid0 += gridDim.x * NGk) {
       if constexpr (G == 64)
@@ -1046,8 +1055,13 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
     // Workgroups are dealt to the 8 XCDs round-robin, and each XCD has its own L2;
     // consecutive groups share window rows, so the first 8*(B/8) blocks are renumbered
     // to give each XCD a contiguous run of groups (a bijection; the tail keeps its order).
+    // B counts the blocks with work: the grid of a launch over a device-side count (a row
+    // list, or the fallback's pixel list) is sized for the most it may hold, and its blocks
+    // past the count exit; renumbering only the live blocks keeps every XCD busy.
     constexpr unsigned NXCD = 8;
-    const unsigned B = gridDim.x, B8 = B / NXCD * NXCD, b = blockIdx.x;
+    const unsigned B = (a.rcount || a.pcount) ? (unsigned)((max(npix, 0) + NGk - 1) / NGk) : gridDim.x;
+    const unsigned B8 = B / NXCD * NXCD, b = blockIdx.x;
+    if (b >= B) return;
     const unsigned lb = b < B8 ? (b % NXCD) * (B8 / NXCD) + b / NXCD : b;
     const int gid0 = (int)lb * NGk;
     if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT, BWD, REF>(a, MP, GREG, gid0, npix);

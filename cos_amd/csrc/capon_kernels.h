@@ -22,8 +22,11 @@ struct CaponLaunch {
   const int* pcount;   // optional device count for plist (overrides npix when set)
   int pbase;           // with pcount: list entries before plist; the launch covers entries
                        // pbase .. *pcount-1 of the whole list, at most npix of them
-  int npix;            // number of pixels (list capacity when pcount is set)
+  int npix;            // number of pixels (list capacity when pcount is set; grid bound with rlist)
   int jlo, jn;         // without plist: output rows jlo .. jlo+jn-1 of every beam (npix = Nx*jn)
+  const int* rlist;    // optional device list of output rows (with rcount, its device length):
+  const int* rcount;   // the launch covers rows rlist[0 .. *rcount-1] of every beam, pixel
+                       // g -> beam g / *rcount, row rlist[g % *rcount]; npix bounds Nx * *rcount
   float* P;            // [npix_total] power 1/(a^H R_L^-1 a), R normalised by K(2Y+1)
   float2* z;           // [npix_total] amplitude w^H g
   float2* w;           // [npix_total][wstride] weights (first L entries)
@@ -49,6 +52,14 @@ struct CaponLaunch {
   // kernels then solve with it instead of building the covariance
   const float2* Rin;
 };
+
+// Row lists of a table-driven frame, built on the device: output row j (0 <= j < nzp) goes
+// to list bucket[L(j + Y)] (L clamped to 0 .. 64), in increasing j; lists [nb][nzp] and
+// counts [nb].  One block; nb <= 8.
+struct RowBuckets {
+  unsigned char b[65];
+};
+hipError_t plan_rows(const int* rowL, int nzp, int Y, RowBuckets bk, int nb, int* lists, int* counts, hipStream_t s);
 
 // Loaded covariance of the pixels of output rows jlo .. jlo+jn-1 (all beams) on the matrix
 // cores (BF16 hi/lo split, FP32 accumulation) into a.R + p*Rs in the reference layout;

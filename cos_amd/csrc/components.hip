@@ -3,6 +3,7 @@
 // BuildR::getR(Nb > 0), and the reference's matrix-beamspace output path.
 #include "cos_device.h"
 #include "components.h"
+#include "capon_kernels.h"
 
 namespace cosk {
 
@@ -12,13 +13,79 @@ __global__ void fill_i32_kernel(int* p, int n, int v) {
 }
 
 // Pixels with any flag bit go to the FP64 list (order is irrelevant: pixels are
-// independent and each writes only its own outputs).
-__global__ void compact_flags_kernel(const uint8_t* flags, int n, int* list, int* count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && flags[i] != 0) {
-    int k = atomicAdd(count, 1);
-    list[k] = i;
+// independent and each writes only its own outputs).  Each lane tests four pixels (one
+// 32-bit load where the four are in range), and each wave reserves its entries with one
+// atomic, ranked by ballot.
+__global__ __launch_bounds__(256) void compact_flags_kernel(const uint8_t* flags, int n, int* list, int* count) {
+  const int i0 = 4 * (blockIdx.x * 256 + threadIdx.x);
+  uint32_t f4 = 0;
+  if (i0 + 3 < n && ((uintptr_t)flags & 3) == 0) {
+    f4 = *reinterpret_cast<const uint32_t*>(flags + i0);
+  } else {
+    for (int u = 0; u < 4; ++u)
+      if (i0 + u < n) f4 |= (uint32_t)flags[i0 + u] << (8 * u);
   }
+  const int lane = threadIdx.x & 63;
+  for (int u = 0; u < 4; ++u) {
+    const bool hit = ((f4 >> (8 * u)) & 0xffu) != 0;
+    const unsigned long long m = __ballot(hit);
+    if (m == 0) continue;
+    int base = 0;
+    if (lane == __ffsll((long long)m) - 1) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, __ffsll((long long)m) - 1);
+    if (hit) list[base + __popcll(m & ((1ull << lane) - 1ull))] = i0 + u;
+  }
+}
+
+// Row lists of a table-driven frame: one block of 1024 threads takes the output rows in
+// chunks of 1024; each row goes to the list of its bucket (the lane shape of its L), in
+// increasing row order, by a ballot prefix within each wave and a sum over the waves.
+__global__ __launch_bounds__(1024) void plan_rows_kernel(const int* rowL, int nzp, int Y, RowBuckets bk, int nb,
+                                                         int* lists, int* counts) {
+  constexpr int NW = 1024 / 64, NB = 8;
+  __shared__ int wtot[NW][NB];
+  __shared__ int base[NB];
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  if (t < NB) base[t] = 0;
+  __syncthreads();
+  for (int j0 = 0; j0 < nzp; j0 += 1024) {
+    const int j = j0 + t;
+    int b = -1;
+    if (j < nzp) {
+      int L = rowL[j + Y];
+      L = L < 0 ? 0 : L > 64 ? 64 : L;
+      b = bk.b[L];
+    }
+    unsigned long long mine = 0;
+    for (int i = 0; i < nb; ++i) {
+      const unsigned long long m = __ballot(b == i);
+      if (b == i) mine = m;
+      if (ln == 0) wtot[wv][i] = __popcll(m);
+    }
+    __syncthreads();
+    if (b >= 0) {
+      int off = base[b];
+      for (int w = 0; w < wv; ++w) off += wtot[w][b];
+      off += __popcll(mine & ((1ull << ln) - 1ull));
+      lists[b * nzp + off] = j;
+    }
+    __syncthreads();
+    if (t < nb) {
+      int n = 0;
+      for (int w = 0; w < NW; ++w) n += wtot[w][t];
+      base[t] += n;
+    }
+    __syncthreads();
+  }
+  if (t < nb) counts[t] = base[t];
+}
+
+hipError_t plan_rows(const int* rowL, int nzp, int Y, RowBuckets bk, int nb, int* lists, int* counts, hipStream_t s) {
+  if (nb < 1 || nb > 8 || nzp < 1) return hipErrorInvalidValue;
+  for (int L = 0; L <= 64; ++L)
+    if (bk.b[L] >= nb) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(plan_rows_kernel, dim3(1), dim3(1024), 0, s, rowL, nzp, Y, bk, nb, lists, counts);
+  return hipGetLastError();
 }
 
 hipError_t fill_i32(int* p, int n, int v, hipStream_t s) {
@@ -30,7 +97,7 @@ hipError_t fill_i32(int* p, int n, int v, hipStream_t s) {
 hipError_t compact_flags(const uint8_t* flags, int n, int* list, int* count, hipStream_t s) {
   hipError_t e = fill_i32(count, 1, 0, s);
   if (e != hipSuccess || n <= 0) return e;
-  hipLaunchKernelGGL(compact_flags_kernel, dim3((n + 255) / 256), dim3(256), 0, s, flags, n, list, count);
+  hipLaunchKernelGGL(compact_flags_kernel, dim3((n + 1023) / 1024), dim3(256), 0, s, flags, n, list, count);
   return hipGetLastError();
 }
 

@@ -45,7 +45,16 @@ __global__ __launch_bounds__(64) void cov_mfma_kernel(const CaponLaunch a, const
   extern __shared__ float2 rows[];
   const int Y = a.Y, TW = 2 * Y + 1, Nzp = a.Nz - 2 * Y;
   const int b = blockIdx.x;
-  const int ix = b / a.jn, j = a.jlo + (b - ix * a.jn);
+  int ix, j;
+  if (a.rlist) {   // rows rlist[0 .. *rcount-1] of every beam; blocks past them exit
+    const int nr = *a.rcount;
+    if (b >= a.Nx * nr) return;
+    ix = b / nr;
+    j = a.rlist[b - ix * nr];
+  } else {
+    ix = b / a.jn;
+    j = a.jlo + (b - ix * a.jn);
+  }
   const int p = ix * Nzp + j, crow = j + Y;
   const int L = a.rowL ? a.rowL[crow] : a.L;
   const int M = a.rowM ? a.rowM[crow] : a.M;
@@ -160,7 +169,7 @@ hipError_t launch_cov_mfma(const CaponLaunch& a, int ntl, int Mmax, int Rs, hipS
   const int N = (2 * a.Y + 1) * Mmax;   // bounds every row's (2Y+1) K
   const size_t lds = (size_t)(2 * a.Y + 1) * MP * sizeof(float2) + (size_t)((N + 7) / 8 * 8) * sizeof(int);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
-  const long blocks = (long)a.Nx * a.jn;
+  const long blocks = a.rlist ? (long)a.npix : (long)a.Nx * a.jn;
   if (blocks == 0) return hipSuccess;
   if (ntl == 1)
     hipLaunchKernelGGL(cov_mfma_kernel<1>, dim3((unsigned)blocks), dim3(64), lds, s, a, MP, Rs);

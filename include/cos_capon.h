@@ -93,7 +93,9 @@ typedef struct {
     const float* steer_sin; /* optional device array of nsteer sin(theta) values */
     int nsteer;
     int graph;           /* nonzero: capture the frame's launches in a hipGraph on the first
-                            call and replay it while descriptor and pointers are unchanged */
+                            call and replay it while descriptor and pointers are unchanged
+                            (the row tables' contents may change: rows are sorted into lane
+                            shapes on the device inside the graph) */
     /* Options of the reference's CPU path (framework/beamformer/capon/getCaponAllC.pyx)
      * and compiled-out GPU variants; 0 selects the reference GPU default. */
     int fb_avg;          /* forward-backward averaging R <- (R + J R^T J)/2 before loading

@@ -835,6 +835,55 @@ def test_graph_replay_matches_eager():
             assert torch.equal(o3[k], e2r[k]), (rep, k)
 
 
+def test_graph_replay_follows_row_tables_changed_in_place():
+    """The lane-shape split of a table-driven frame is planned on the device inside the
+    frame graph, so a replay after the caller rewrites the L, M and d tables in place
+    (same pointers, same Lmax) computes the new frame: rows moved between lane shapes
+    (L 12 <-> 40 <-> 60) and into the fallback match an eager run on the new tables."""
+    import torch
+    from cos_amd import CaponPlan
+    say("graph replay, tables changed in place")
+    Nx, Nz, Mmax, Y = 4, 300, 128, 2
+    x = torch.from_numpy(uniform_cube(44, Nx, Nz, Mmax)).cuda()
+    rng = np.random.default_rng(45)
+
+    def tables(seed):
+        r = np.random.default_rng(seed)
+        L = r.choice([12, 24, 40, 60], Nz).astype(np.int32)
+        M = np.minimum(2 * L + r.integers(0, 8, Nz), Mmax).astype(np.int32)
+        off = ((Mmax - M) // 2).astype(np.int32)
+        d = np.where(r.random(Nz) < 0.1, 1e-6, 0.01).astype(np.float32)
+        return L, M, off, d
+
+    rows = {k: torch.from_numpy(v).cuda() for k, v in zip(("L", "M", "off", "d"), tables(1))}
+    plan = CaponPlan()
+    out = {k: torch.empty(s, dtype=t, device="cuda") for k, s, t in
+           (("P", (Nx * (Nz - 2 * Y),), torch.float32), ("z", (Nx * (Nz - 2 * Y),), torch.complex64),
+            ("w", (Nx * (Nz - 2 * Y), 60), torch.complex64))}
+    for seed in (1, 2, 3):
+        for k, v in zip(("L", "M", "off", "d"), tables(seed)):
+            rows[k].copy_(torch.from_numpy(v))                     # in place: same pointers
+        for v in out.values():
+            v.zero_()
+        plan.run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", out=out, graph=True)
+        torch.cuda.synchronize()
+        n_graph = plan.fallback_count()
+        eager = CaponPlan().run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback")
+        torch.cuda.synchronize()
+        for k in out:
+            assert torch.equal(out[k][:, :eager[k].shape[-1]] if k == "w" else out[k], eager[k]), (seed, k)
+        assert bool(torch.isfinite(out["P"]).all())
+        Lh = rows["L"].cpu().numpy()
+        c = np.arange(Nz - 2 * Y) + Y
+        assert n_graph == Nx * int((rows["d"].cpu().numpy()[c] < 1e-3).sum()), seed
+        sel = np.arange(0, Nx * (Nz - 2 * Y), 5)
+        ref = O.capon_rows(x.cpu().numpy().astype(np.complex128), rows["d"].cpu().numpy().astype(np.float64), Lh,
+                           rows["M"].cpu().numpy(), rows["off"].cpu().numpy(), Y,
+                           pixels=(sel // (Nz - 2 * Y), sel % (Nz - 2 * Y)))
+        assert rel(out["P"].cpu().numpy()[sel], ref["P"]).max() <= 1e-4, seed
+    del rng
+
+
 def test_graph_replay_after_plan_buffers_grow():
     """Graph of frame A (row tables, fallback on), then an eager larger frame B on the same
     plan, whose fallback buffers (flags, list, count) must grow, then graph A again: the
