@@ -156,14 +156,16 @@ class CaponPlan:
         averaged covariance; z_mode: "amplitude" (w^H g) or "power" (power Capon);
         p_mode: "loaded" (1/(a^H R_L^-1 a)) or "unloaded" (|w^H R_hat w|).  refine: one step
         of iterative refinement on the FP32 paths (residual from the snapshots).  cov_mfma:
-        covariance on the matrix cores (BF16 hi/lo split), then the FP32 solve from it
-        (L <= 64, row tables allowed, default options).  Returns a dict of output tensors.
+        True: covariance on the matrix cores (BF16 hi/lo split) into R, then the FP32 solve
+        from it (L <= 64, row tables allowed, default options); "fused": per-snapshot Grams
+        on the matrix cores inside the fused kernel where it applies (16 < L <= 32, windows
+        of up to five rows, default options).  Returns a dict of output tensors.
         """
         ds, o, ptr, wstride, stream = self._prepare(
             x, d, L, Yavg, M=M, off=off, precision=precision, want=want, rows=rows, steer_sin=steer_sin,
             cond_thresh=cond_thresh, out=out, stream=stream, graph=graph, fb_avg=fb_avg, z_mode=z_mode,
             p_mode=p_mode, refine=refine)
-        ds.cov_mfma = int(bool(cov_mfma))
+        ds.cov_mfma = 2 if cov_mfma in ("fused", 2) else int(bool(cov_mfma))
         check(lib().cos_capon_run(self._h, C.byref(ds), x.data_ptr(), ptr("P"), ptr("z"), ptr("w"),
                                   wstride, ptr("R"), ptr("flags"), stream))
         return o

@@ -311,9 +311,12 @@ int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
     return fail(hipErrorInvalidValue, "steering takes the default z and P definitions");
   if (ds.refine && (ds.precision == COS_PREC_FP64 || ds.fb_avg))
     return fail(hipErrorInvalidValue, "refinement applies to the FP32 paths without forward-backward averaging");
-  if (ds.cov_mfma && (ds.precision == COS_PREC_FP64 || ds.nsteer > 0 || ds.fb_avg || ds.z_mode || ds.p_mode ||
-                      ds.refine))
+  if (ds.cov_mfma < 0 || ds.cov_mfma > 2) return fail(hipErrorInvalidValue, "bad cov_mfma");
+  if (ds.cov_mfma == 1 && (ds.precision == COS_PREC_FP64 || ds.nsteer > 0 || ds.fb_avg || ds.z_mode ||
+                           ds.p_mode || ds.refine))
     return fail(hipErrorInvalidValue, "the MFMA covariance takes an FP32 path and the default options");
+  if (ds.cov_mfma == 2 && ds.precision == COS_PREC_FP64)
+    return fail(hipErrorInvalidValue, "the fused MFMA covariance takes an FP32 path");
   return 0;
 }
 
@@ -343,6 +346,7 @@ int enqueue_frame(const FrameCtx& ctx, const cos_capon_desc& ds, const cos_cf32*
   a.R = (float2*)R;
   a.Rstride = ds.L * ds.L;
   a.refine = ds.refine ? 1 : 0;
+  a.mfma_gram = ds.cov_mfma == 2 ? 1 : 0;
   const char* why = nullptr;
   hipError_t e;
   const int Nzp = ds.Nz - 2 * ds.Yavg;
@@ -356,7 +360,7 @@ int enqueue_frame(const FrameCtx& ctx, const cos_capon_desc& ds, const cos_cf32*
     c.npix = ds.Nx * Nzp;
     c.Lcap = ctx.bu.cap[b];
   };
-  if (ds.cov_mfma) {
+  if (ds.cov_mfma == 1) {
     // covariance on the matrix cores into R (the caller's, or the plan's; Lmax^2 per pixel),
     // then the solve reads it back; one launch per row run, with one or four 32 x 32 tiles
     cosk::CaponLaunch c = a;
@@ -448,7 +452,7 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   const bool fb = fallback_needed(ds);
   const int npix = ds.Nx * (ds.Nz - 2 * ds.Yavg);
   hipError_t e;
-  if (ds.cov_mfma && !R) {
+  if (ds.cov_mfma == 1 && !R) {
     const bool tab = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
     const size_t Lm = tab ? ds.Lmax : ds.L;
     if ((e = plan->R.ensure((size_t)npix * Lm * Lm * sizeof(cos_cf32))) != hipSuccess)

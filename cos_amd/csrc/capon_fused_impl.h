@@ -36,6 +36,7 @@
 #pragma once
 #include "cos_device.h"
 #include "capon_kernels.h"
+#include "mfma_gram.h"
 
 namespace cosk {
 
@@ -103,7 +104,10 @@ template <typename T, int G, int CPL, int TM> struct OccHint {
                                               : (G == 16 && CPL == 1 ? 2 : 1);
 };
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0, bool REF = false>
+// MG: a wave whose four pixels are consecutive rows of one beam (L <= 32) builds their
+// covariances from per-snapshot Grams on the matrix cores (mfma_gram.h) instead of the
+// displacement recursion; the launch's other waves keep the recursion.
+template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0, bool REF = false, bool MG = false>
 __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, const int GREG,
                                             const int gid0, const int npix) {
   using Gp = Grp<G>;
@@ -156,17 +160,25 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // windows overlap in NG + 2Y rows, which are staged once; group q then uses rows
   // q .. q+2Y.  Otherwise each group stages its own 2Y+1 rows.
   const int NR = NG + TW - 1;
-  bool shared = false;
+  bool shared = false, mg = false;
   if constexpr (NG > 1) {
     const int p0 = __builtin_amdgcn_readfirstlane(p);
     const int L0 = __builtin_amdgcn_readfirstlane(L), M0 = __builtin_amdgcn_readfirstlane(M);
     const int o0 = __builtin_amdgcn_readfirstlane(off);
     const int j0 = p0 - (p0 / Nzp) * Nzp;
     const bool mine = active && p == p0 + q && L == L0 && M == M0 && off == o0 && j0 + NG <= Nzp;
-    shared = __all(mine) && (NR * MP + NR * LMAX * (int)(sizeof(cplx<T>) / sizeof(float2)) <= NG * GREG);
+    const bool consec = __all(mine);
+    shared = consec && (NR * MP + NR * LMAX * (int)(sizeof(cplx<T>) / sizeof(float2)) <= NG * GREG);
+    if constexpr (MG) mg = consec && L <= 32 && mfma_lds(M - L + 1, NR) <= NG * GREG;
   }
   float2* xs = shared ? xs_all + q * MP : xs_all + q * GREG;
-  if (shared) {
+  const float2* const wrow0 = a.x + ((size_t)(p - q) / Nzp * a.Nz + (j - q)) * a.Mstride + off;   // (shared, mg)
+  int pexp = 0;   // with mg: the wave's rows were scaled by 2^pexp, P by 2^(2 pexp)
+  if (mg) {
+    if constexpr (MG)
+      pexp = mfma_stage(reinterpret_cast<unsigned*>(xs_all), wrow0, a.Mstride, __builtin_amdgcn_readfirstlane(M), NR,
+                        mfma_mpf(__builtin_amdgcn_readfirstlane(M - L + 1)));
+  } else if (shared) {
     const float2* src0 = a.x + ((size_t)(p - q) / Nzp * a.Nz + (j - q)) * a.Mstride + off;
     for (int t = 0; t < NR; ++t) {
       const float2* src = src0 + (size_t)t * a.Mstride;
@@ -187,12 +199,68 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 
   cplx<T> A[NS];
 
+  if constexpr (MG) {
+    if (mg) {
+      // pixel q's window sum of the row Grams into the transfer buffer, then into the
+      // registers of group q
+      // (L and M are the same in the wave's four groups)
+      const unsigned* const F = reinterpret_cast<const unsigned*>(xs_all);
+      const int K0 = __builtin_amdgcn_readfirstlane(M - L + 1), MPF = mfma_mpf(K0), NSTEP = (K0 + 7) >> 3;
+      float2* const Tb = xs_all + 2 * NR * MPF;
+      auto take = [&](const f32x16& wr, const f32x16& wi, int qq) {
+        __syncthreads();   // the previous window has been read
+        window_store(Tb, wr, wi);
+        __syncthreads();
+        if (q == qq) {
+          static_for<0, CPL>([&](auto S_) {
+            constexpr int S = decltype(S_)::value;
+            static_for<0, G * (S + 1)>([&](auto R_) {
+              constexpr int R = decltype(R_)::value;
+              const float2 v = Tb[R * 32 + cs[S]];
+              A[slot<G, S, R>()] = {(T)v.x, (T)v.y};
+            });
+          });
+        }
+      };
+      const f32x16 zero = {};
+      f32x16 ur, ui, xr, xi, sr, si;
+      if (TW == 5) {
+        // the four windows of five rows out of eight from ten Gram passes, with at most
+        // two accumulator pairs live once the registers of A hold a window:
+        // U = G3 + G4, X = U + G2 + G1, S0 = X + G0, S1 = X + G5,
+        // Y = U + G5 + G6 (in U), S2 = Y + G2, S3 = Y + G7
+        gram_pass(F, 3, MPF, NSTEP, K0, zero, zero, ur, ui);
+        gram_pass(F, 4, MPF, NSTEP, K0, ur, ui, ur, ui);
+        gram_pass(F, 2, MPF, NSTEP, K0, ur, ui, xr, xi);
+        gram_pass(F, 1, MPF, NSTEP, K0, xr, xi, xr, xi);
+        gram_pass(F, 0, MPF, NSTEP, K0, xr, xi, sr, si);
+        take(sr, si, 0);
+        gram_pass(F, 5, MPF, NSTEP, K0, xr, xi, xr, xi);
+        take(xr, xi, 1);
+        gram_pass(F, 5, MPF, NSTEP, K0, ur, ui, ur, ui);
+        gram_pass(F, 6, MPF, NSTEP, K0, ur, ui, ur, ui);
+        gram_pass(F, 2, MPF, NSTEP, K0, ur, ui, xr, xi);
+        take(xr, xi, 2);
+        gram_pass(F, 7, MPF, NSTEP, K0, ur, ui, ur, ui);
+        take(ur, ui, 3);
+      } else {
+        for (int qq = 0; qq < NG; ++qq) {
+          gram_pass(F, qq, MPF, NSTEP, K0, zero, zero, sr, si);
+          for (int t = 1; t < TW; ++t) gram_pass(F, qq + t, MPF, NSTEP, K0, sr, si, sr, si);
+          take(sr, si, qq);
+        }
+      }
+    }
+  }
+
   // ---------------- covariance S (unnormalised) ----------------
   // (or the loaded S_L of every pixel given in a.Rin, runtime-window FP32 kernels only)
   constexpr bool FR = OPTS && !REF && std::is_same<T, float>::value;
   bool from_r = false;
   if constexpr (FR) from_r = a.Rin != nullptr;
-  if (from_r) {
+  if (mg) {
+    // built above
+  } else if (from_r) {
     const float2* Rp = a.Rin + (size_t)p * a.Rstride;
     static_for<0, CPL>([&](auto S_) {
       constexpr int S = decltype(S_)::value;
@@ -353,15 +421,26 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 
   // ---------------- trace, loading, padding ----------------
   // tr(S) = sum_t sum_m |x_t[m]|^2 * #{(k, l): k + l = m, k < K, l < L}
-  T trp = T(0);
-  for (int t = 0; t < TW; ++t) {
-    for (int m = g; m < M; m += G) {
-      const cplx<T> v = ldx<T>(xs + t * MP + m);
-      const int cnt = min(L - 1, m) - max(0, m - K + 1) + 1;
-      trp = fma((T)cnt, fma(v.x, v.x, v.y * v.y), trp);
+  T trS;
+  if (mg) {
+    // the diagonal of the Gram sum (L is the same in the wave's four groups)
+    T tr = T(0);
+    static_for<0, LMAX>([&](auto K_) {
+      constexpr int k = decltype(K_)::value;
+      if (k < L) tr += Gp::template bcast<k % G>(A[slot<G, k / G, k>()].x);
+    });
+    trS = tr;
+  } else {
+    T trp = T(0);
+    for (int t = 0; t < TW; ++t) {
+      for (int m = g; m < M; m += G) {
+        const cplx<T> v = ldx<T>(xs + t * MP + m);
+        const int cnt = min(L - 1, m) - max(0, m - K + 1) + 1;
+        trp = fma((T)cnt, fma(v.x, v.x, v.y * v.y), trp);
+      }
     }
+    trS = Gp::sum(trp);
   }
-  const T trS = Gp::sum(trp);
   const T delta = from_r ? T(0) : (T)dload * trS / (T)L;   // a.Rin is loaded already
   const T invK = T(1) / (T)K;
   const bool steered = a.nsteer > 0;
@@ -374,11 +453,20 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       constexpr int S = decltype(S_)::value;
       cplx<T> acc = {T(0), T(0)};
       if (!steered && cs[S] < L) {
-        const float2* xc = xs + Y * MP + cs[S];
-        for (int k = 0; k < K; ++k) {
-          const cplx<T> v = ldx<T>(xc + k);
-          acc.x += v.x;
-          acc.y += v.y;
+        if (MG && mg) {   // the rows are not in LDS: the centre row from memory
+          const float2* xc = wrow0 + (size_t)(q + Y) * a.Mstride + cs[S];
+          for (int k = 0; k < K; ++k) {
+            const cplx<T> v = ldx<T>(xc + k);
+            acc.x += v.x;
+            acc.y += v.y;
+          }
+        } else {
+          const float2* xc = xs + Y * MP + cs[S];
+          for (int k = 0; k < K; ++k) {
+            const cplx<T> v = ldx<T>(xc + k);
+            acc.x += v.x;
+            acc.y += v.y;
+          }
         }
       }
       gS[S] = {acc.x * invK, acc.y * invK};
@@ -663,8 +751,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 
     const size_t po = (size_t)p * nrhs + ia;
     const float nanv = __builtin_nanf("");
-    if (!REF && active && g == 0 && a.P && (!OPTS || a.pmode == 0))
-      a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
+    if (!REF && active && g == 0 && a.P && (!OPTS || a.pmode == 0)) {
+      float pv = (float)(T(1) / (norm * accP));
+      if constexpr (MG) pv = ldexpf(pv, -2 * pexp);   // the solve ran on 2^(2 pexp) S
+      a.P[po] = valid ? pv : nanv;
+    }
 
     if (want_y) {
       cplx<T> y[CPL];
@@ -1034,7 +1125,8 @@ __device__ __noinline__ void capon_group_call(const CaponLaunch& a, const int MP
   capon_group<T, G, CPL, TM, DIRECT, BWD, REF>(a, MP, GREG, gid0, npix);
 }
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false,
+          bool MG = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
                           amdgpu_waves_per_eu(BWD ? 1 : (REF && G == 64) ? 2 : OccHint<T, G, CPL, TM>::value)))
 void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
@@ -1064,12 +1156,13 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
     if (b >= B) return;
     const unsigned lb = b < B8 ? (b % NXCD) * (B8 / NXCD) + b / NXCD : b;
     const int gid0 = (int)lb * NGk;
-    if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT, BWD, REF>(a, MP, GREG, gid0, npix);
+    if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT, BWD, REF, MG>(a, MP, GREG, gid0, npix);
   }
 }
 
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false,
+          bool MG = false>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   constexpr int NG = 64 / G, LMAX = G * CPL;
   const int MP = ((Mmax + LMAX + 4 + 3) / 4) * 4;   // zero padding covers every lag read
@@ -1092,7 +1185,8 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   int blocks = (a.npix + NG - 1) / NG;
   if (PERSIST && blocks > 2048) blocks = 2048;   // list length known only on the device
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST, BWD, REF>), dim3(blocks), dim3(64), lds, s, a, MP, GREG);
+  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST, BWD, REF, MG>), dim3(blocks), dim3(64), lds, s, a, MP,
+                     GREG);
   return hipGetLastError();
 }
 

@@ -51,6 +51,9 @@ struct CaponLaunch {
   // conj(S_L[r][c]) for the pixel's L, pixel stride Rstride): the runtime-window FP32
   // kernels then solve with it instead of building the covariance
   const float2* Rin;
+  // nonzero: the 16-lane, two-column FP32 kernel builds the covariances of its waves from
+  // per-snapshot Grams on the matrix cores (windows of up to five rows, no R output)
+  int mfma_gram;
 };
 
 // Row lists of a table-driven frame, built on the device: output row j (0 <= j < nzp) goes

@@ -1,0 +1,154 @@
+// Per-snapshot Gram matrices on the matrix cores, for the fused Capon kernel.
+//
+// A wave of the fused kernel holds four pixels that are consecutive range rows of one beam
+// (16 lanes each).  Their windows cover NR = 4 + 2Y snapshot rows, and pixel q's covariance
+// is the sum of the Grams of rows q .. q+2Y:
+//   S_q = sum_{t=q}^{q+2Y} G_t,   G_t[i][j] = sum_{k<K} x_t[k+i] conj(x_t[k+j])
+// (framework/beamformer/capon/RealTimeCapon/buildR_kernel.cu:45-161 forms the same S per
+// pixel).  Each G_t is a 32 x 32 complex product over the row's K subarrays, computed once
+// per wave on v_mfma_f32_32x32x16_f16 and shared by every window that contains the row,
+// instead of the displacement recursion on the VALU.
+//
+// Precision.  One power-of-two scale per wave, 2^s with s chosen so that the largest
+// |Re|, |Im| of the wave's rows lies in [2^7, 2^8), keeps every value inside the FP16
+// range; each scaled value is split x = hi + lo with hi = fp16(x), lo = fp16(x - hi)
+// (round to nearest even, 22 significant bits together), and a Gram is hi*hi + hi*lo +
+// lo*hi accumulated in FP32 (the omitted lo*lo is 2^-22 of the product).  The scale is
+// exact: the solve runs on 2^(2s) S, whose weights and amplitude equal those of S, and its
+// power is scaled back by 2^(-2s).
+//
+// Operand layout (v_mfma_f32_32x32x16_f16, wave64): lane l supplies A[i = l%32][k] and
+// B[k][j = l%32] for k = 8(l/32) + 0..7, and holds D[i][j] for j = l%32,
+// i = 8(v/4) + 4(l/32) + v%4 in accumulator element v < 16.  With the inner index
+// k = 2u + part over four subarrays n = 8s + 4(l/32) + u of step s, the real part of the
+// Gram is Q^T Q and the imaginary part P^T Q, Q[k][c] = (Re, Im)(x[n + c]) and
+// P[k][c] = (Im, -Re)(x[n + c]): each lane reads the four consecutive elements
+// x[8s + 4(l/32) + c .. +3] of its column c, for A and B alike.  The split rows are staged
+// in LDS once per wave as four planes per row, Qhi, Qlo, Phi, Plo (one 32-bit pair of
+// halves per element), so each operand is one contiguous 16-byte read into consecutive
+// registers: a step costs four LDS reads and six MFMAs, and no VALU work.
+#pragma once
+#include "cos_device.h"
+
+namespace cosk {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Split-row length per snapshot row (elements): every read x[n + c] with n < 8 ceil(K/8),
+// c < 32 lies below it.
+__device__ __forceinline__ int mfma_mpf(int K) { return 8 * ((K + 7) >> 3) + 32; }
+// LDS the path needs for a wave, in float2 units: the split rows [NR][4 planes][MPF] of
+// 4 bytes and the 32 x 32 window buffer.
+__device__ __forceinline__ int mfma_lds(int K, int NR) { return NR * mfma_mpf(K) * 2 + 1024; }
+
+__device__ __forceinline__ unsigned pack_h2(_Float16 lo, _Float16 hi) {
+  return (unsigned)__builtin_bit_cast(unsigned short, lo) | ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
+}
+
+// Qhi, Qlo, Phi, Plo of one scaled element (re, im) into the four planes of its row
+__device__ __forceinline__ void split_f16(unsigned* row, int MPF, int m, float re, float im) {
+  const _Float16 rh = (_Float16)re, ih = (_Float16)im;
+  const _Float16 rl = (_Float16)(re - (float)rh), il = (_Float16)(im - (float)ih);
+  row[m] = pack_h2(rh, ih);
+  row[MPF + m] = pack_h2(rl, il);
+  row[2 * MPF + m] = pack_h2(ih, -rh);
+  row[3 * MPF + m] = pack_h2(il, -rl);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Stages the wave's NR rows (src0 + t Mstride, M elements each) as split FP16 planes
+// F[t][plane][m], m < MPF, zero past M.  Returns the exponent e of the power scale applied
+// to the rows: the data's S is 2^(-2e) times the Gram sums built from F.
+__device__ __forceinline__ int mfma_stage(unsigned* F, const float2* src0, int Mstride, int M, int NR, int MPF) {
+  float mx = 0.f;
+  for (int t = 0; t < NR; ++t)
+    for (int m = threadIdx.x; m < M; m += 64) {
+      const float2 v = src0[(size_t)t * Mstride + m];
+      mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+  mx = wave_max(mx);
+  int ex = 0;
+  (void)frexpf(mx, &ex);                       // mx < 2^ex
+  const int e = (mx > 0.f && mx <= 3.0e38f) ? 8 - ex : 0;
+  const float sc = ldexpf(1.f, e);
+  for (int t = 0; t < NR; ++t)
+    for (int m = threadIdx.x; m < MPF; m += 64) {
+      float2 v = make_float2(0.f, 0.f);
+      if (m < M) v = src0[(size_t)t * Mstride + m];
+      split_f16(F + 4 * t * MPF, MPF, m, v.x * sc, v.y * sc);
+    }
+  return e;
+}
+
+__device__ __forceinline__ half8 as_h8(u32x4 v) { return __builtin_bit_cast(half8, v); }
+
+__device__ __forceinline__ u32x4 ld4(const unsigned* p) {
+  u32x4 v;
+  __builtin_memcpy(&v, p, sizeof(v));
+  return v;
+}
+
+// One step: the lane's four subarrays n = 8 s8 + 4h + u into (r, im); in the last step
+// (MASK) the B operands of subarrays n >= K are zero.
+template <bool MASK>
+__device__ __forceinline__ void gram_step(const unsigned* Ft, int MPF, int s8, int K, const f32x16& cr,
+                                          const f32x16& ci, f32x16& r, f32x16& im) {
+  const unsigned* e = Ft + 8 * s8;
+  u32x4 qh = ld4(e), ql = ld4(e + MPF);
+  const u32x4 ph = ld4(e + 2 * MPF), pl = ld4(e + 3 * MPF);
+  if constexpr (MASK) {
+    const int nv = K - (8 * s8 + 4 * ((threadIdx.x >> 5) & 1));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      qh[u] = u < nv ? qh[u] : 0u;
+      ql[u] = u < nv ? ql[u] : 0u;
+    }
+  }
+  r = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(qh), as_h8(qh), cr, 0, 0, 0);
+  im = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(ph), as_h8(qh), ci, 0, 0, 0);
+  r = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(qh), as_h8(ql), r, 0, 0, 0);
+  im = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(ph), as_h8(ql), im, 0, 0, 0);
+  r = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(ql), as_h8(qh), r, 0, 0, 0);
+  im = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(pl), as_h8(qh), im, 0, 0, 0);
+}
+
+// D = C + G_t: the Gram of staged row t (K subarrays, NSTEP = ceil(K/8) steps, both
+// wave-uniform), real part in dr, imaginary in di.  The first step reads C, so a chain
+// that starts from another accumulator copies nothing.
+__device__ __forceinline__ void gram_pass(const unsigned* F, int t, int MPF, int NSTEP, int K, const f32x16& cr,
+                                          const f32x16& ci, f32x16& dr, f32x16& di) {
+  const int c = threadIdx.x & 31, h = (threadIdx.x >> 5) & 1;
+  const unsigned* Ft = F + 4 * t * MPF + 4 * h + c;
+  const bool ragged = 8 * NSTEP > K;
+  f32x16 r, im;
+  if (NSTEP == 1) {
+    if (ragged) gram_step<true>(Ft, MPF, 0, K, cr, ci, r, im);
+    else gram_step<false>(Ft, MPF, 0, K, cr, ci, r, im);
+  } else {
+    gram_step<false>(Ft, MPF, 0, K, cr, ci, r, im);
+    for (int s8 = 1; s8 < NSTEP - 1; ++s8) gram_step<false>(Ft, MPF, s8, K, r, im, r, im);
+    if (ragged) gram_step<true>(Ft, MPF, NSTEP - 1, K, r, im, r, im);
+    else gram_step<false>(Ft, MPF, NSTEP - 1, K, r, im, r, im);
+  }
+  dr = r;
+  di = im;
+}
+
+// The window (real sr, imaginary si, MFMA layout) into the 32 x 32 buffer Tb[i][j].
+__device__ __forceinline__ void window_store(float2* Tb, const f32x16& sr, const f32x16& si) {
+  const int c = threadIdx.x & 31, h = (threadIdx.x >> 5) & 1;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int i = 8 * (v / 4) + 4 * h + (v % 4);
+    Tb[i * 32 + c] = make_float2(sr[v], si[v]);
+  }
+}
+
+}  // namespace cosk

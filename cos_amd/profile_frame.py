@@ -1,0 +1,34 @@
+"""Runs the per-pixel near-field frame (256 beams x 2048 ranges x 64 elements, L = 32,
+temporal window +/-2, loading 0.01 trace/L; seeded speckle and scatterers) a few times on
+cuda:0, for a profiler to observe the kernels:
+
+    python -m cos_amd.profile_frame [--cov recursion|fused|split] [--frames N] [--refine]
+
+--cov selects the covariance: the FP32 displacement recursion inside the fused kernel (the
+default), per-snapshot Grams on the matrix cores inside the fused kernel (fused), or the
+separate matrix-core covariance into R followed by the solve (split).
+"""
+from __future__ import annotations
+
+import argparse
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cov", default="recursion", choices=["recursion", "fused", "split"])
+    ap.add_argument("--frames", type=int, default=3)
+    ap.add_argument("--refine", action="store_true")
+    args = ap.parse_args()
+    import torch
+    from cos_amd import CaponPlan, synth
+    x = torch.from_numpy(synth.config2(256, 2048, 64, seed=2)).cuda()
+    plan = CaponPlan()
+    mode = {"recursion": False, "fused": "fused", "split": True}[args.cov]
+    for _ in range(args.frames):
+        plan.run(x, 0.01, 32, 2, cov_mfma=mode, refine=args.refine)
+    torch.cuda.synchronize()
+    print(f"{args.frames} frames, covariance: {args.cov}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -105,10 +105,16 @@ typedef struct {
     int refine;          /* nonzero: one step of iterative refinement on the FP32 paths, the
                             residual a - R_L y taken from the snapshots themselves (a = 1 or
                             the steering vector; no forward-backward averaging) */
-    int cov_mfma;        /* nonzero: covariance on the matrix cores (BF16 hi/lo split, FP32
+    int cov_mfma;        /* 0: covariance by the FP32 displacement recursion;
+                            1: covariance on the matrix cores (BF16 hi/lo split, FP32
                             accumulation) into R (or a plan buffer when R is NULL), then the
                             FP32 solve from it; L <= 64, row tables allowed (R then a plan
-                            buffer of Lmax*Lmax per pixel), default options */
+                            buffer of Lmax*Lmax per pixel), default options;
+                            2: per-snapshot Grams on the matrix cores inside the fused kernel
+                            (FP16 hi/lo split of power-of-two scaled rows, FP32 accumulation),
+                            shared by the windows of a wave's four consecutive pixels, where
+                            it applies (16 lanes x 2 columns, 16 < L <= 32, windows of up to
+                            five rows, default options, no R output); the recursion elsewhere */
 } cos_capon_desc;
 
 enum {

@@ -134,6 +134,35 @@ def test_cfg2_slice_refined_power(cfg2_slice):
     assert ze.max() <= 1e-4
 
 
+@pytest.mark.parametrize("M,L,Y,Nx,Nz", [(64, 32, 2, 2, 40), (48, 24, 1, 3, 37), (40, 20, 0, 2, 30), (80, 32, 2, 2, 44),
+                                       (33, 17, 2, 1, 21), (64, 32, 2, 3, 43)])
+def test_fused_mfma_covariance_matches_oracle(M, L, Y, Nx, Nz):
+    """The covariance from per-snapshot Grams on the matrix cores inside the fused kernel
+    (FP16 hi/lo split of power-of-two scaled rows): P, w and z against the FP64 oracle, on
+    shapes whose waves take the matrix-core path (four consecutive rows) and, where the
+    row count is not a multiple of four, the recursion at the beam ends."""
+    say("fused mfma covariance", M, L, Y, Nx, Nz)
+    x = uniform_cube(M * 7 + L + Y, Nx, Nz, M)
+    o, _ = _plan_run(x, 0.05, L, Y, cov_mfma="fused")
+    ref = O.capon(x.astype(np.complex128), 0.05, L, Y)
+    eP = rel(o["P"], ref["P"])
+    we = np.linalg.norm(o["w"] - ref["w"], axis=1) / np.linalg.norm(ref["w"], axis=1)
+    ez = np.abs(o["z"] - ref["z"]).max() / np.abs(ref["z"]).max()
+    print(f"P {eP.max():.2e} w {we.max():.2e} z {ez:.2e}", flush=True)
+    assert eP.max() <= 1e-5 and we.max() <= 1e-5 and ez <= 1e-5
+
+
+def test_fused_mfma_covariance_headline_slice(cfg2_slice):
+    """The 64-element, L = 32 slice (speckle and 30 dB scatterers spanning 2^5 in amplitude
+    within a wave's rows) through the matrix-core covariance: power within 1e-5 of FP64."""
+    say("fused mfma covariance cfg2 slice", 8, 2048, 64, 32)
+    x, ref = cfg2_slice
+    o, _ = _plan_run(x, 0.01, 32, 2, cov_mfma="fused")
+    e = rel(o["P"], ref["P"].ravel())
+    print(f"P rel err max {e.max():.3e}", flush=True)
+    assert e.max() <= 1e-5
+
+
 def test_refinement_on_ill_conditioned_covariance():
     """d = 1e-4 (condition bound 1 + L/d = 3.2e5, below the fallback threshold): the FP32
     solve loses digits; one refinement step must bring power and weights closer to FP64."""
