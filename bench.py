@@ -89,7 +89,12 @@ def cpu_baseline(x, budget_s=12.0):
     t0 = time.perf_counter()
     oracle.c_capon(x, D, L, Y, ix=(0, 2), nthreads=1, want_w=True)
     t1 = time.perf_counter() - t0
-    return {"value": rate, "unit": "pixels/s", "cores": threads, "kind": "port",
+    model = "unknown"
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    return {"value": rate, "unit": "pixels/s", "cores": threads, "cpu_model": model, "kind": "port",
             "sample": f"{frames} whole frame(s) (FP64, OpenMP over pixels) in {t:.1f} s",
             "single_thread_value": 2 * (NZ - 2 * Y) / t1,
             "single_thread_sample": f"2 beams x {NZ - 2 * Y} rows on 1 thread, {t1:.1f} s"}

@@ -36,6 +36,28 @@ def _timed(torch, fn, n, stream=None):
     return e0.elapsed_time(e1) / n
 
 
+def _host():
+    """Threads the CPU legs use and the host's CPU model."""
+    import os
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def _cpu_line(value, unit, sample, kind="port", extra=None):
+    cores, model = _host()
+    d = {"value": value, "unit": unit, "cores": cores, "cpu_model": model, "kind": kind, "sample": sample}
+    if extra:
+        d.update(extra)
+    return d
+
+
 def _flops(L, K):
     return 8 * L * L * K + 4 * L * L + (4 * L ** 3) // 3 + 8 * L * L + 32 * L
 
@@ -119,6 +141,28 @@ def aperture(args):
     line["mfma_covariance"] = {"ms_per_frame": round(ms_mf, 3), "vs_all_fp64_P_rel_err_max": float(em.max()),
                                "vs_all_fp64_P_rel_err_mean": float(em.mean()),
                                "path": "BF16 hi/lo MFMA covariance per row run -> R in HBM -> FP32 solves"}
+    # CPU leg: the oracle's C FP64 port over every output row of 8 beams, one call per run of
+    # rows with the same (L, M, first element, d); pixels/s of that subset
+    from oracle import c_capon
+    xs8 = x[:8].cpu().numpy()
+    Nzp = Nz - 2 * Y
+    c = np.arange(Nzp) + Y
+    key = np.stack([Lz[c], Mz[c], off[c], dz[c].view(np.int32)], axis=1)
+    cores, _ = _host()
+    t0 = time.perf_counter()
+    jj = 0
+    while jj < Nzp:
+        j2 = jj + 1
+        while j2 < Nzp and (key[j2] == key[jj]).all():
+            j2 += 1
+        Lr, Mr, orr = int(Lz[c[jj]]), int(Mz[c[jj]]), int(off[c[jj]])
+        c_capon(xs8[:, :, orr:orr + Mr], float(dz[c[jj]]), Lr, Y, j=(jj, j2), want_w=True, nthreads=cores)
+        jj = j2
+    tc = time.perf_counter() - t0
+    line["cpu_baseline"] = _cpu_line(8 * Nzp / tc, "pixels/s",
+                                     f"8 of {Nx} beams x all {Nzp} rows (FP64 C port, OpenMP over pixels) in "
+                                     f"{tc:.1f} s; the frame's rate is the same per pixel")
+    line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)
     return line
 
 
@@ -157,6 +201,16 @@ def beamspace(args):
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_pixel": bpx,
                         "fp32_tflops": round(fpx * npx / (ms * 1e-3) / 1e12, 2), "flop_per_pixel": fpx}
     line["accuracy"] = {"P_rel_err_max_128px": float(e.max()), "w_rel_err_max_128px": float(ew.max())}
+    # CPU leg: the numpy FP64 restatement (sliding DFT, Nb x Nb Gram, Cholesky), vectorised
+    # over pixels, on 2 beams x 2048 rows
+    xc = x[:2, :2048 + 2 * Y].cpu().numpy().astype(np.complex128)
+    t0 = time.perf_counter()
+    O.capon_beamspace(xc, d, L, Y, Nb)
+    tc = time.perf_counter() - t0
+    line["cpu_baseline"] = _cpu_line(2 * 2048 / tc, "pixels/s",
+                                     f"2 of {Nx} beams x 2048 rows, numpy FP64 restatement in {tc:.1f} s")
+    line["cpu_baseline"]["cores"] = 1
+    line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)
     return line
 
 
@@ -197,6 +251,18 @@ def steering(args):
     line["accuracy"] = {"P_rel_err_max": float(e.max()), "w_rel_err_max": float(ew.max())}
     line["unrefined"] = {"ms": round(ms0, 4), "powers_per_s": npow / (ms0 * 1e-3), "P_rel_err_max": float(e0.max()),
                          "w_rel_err_max": float(ew0.max())}
+    # CPU leg: the reference's way to steer, one drop-in call per angle on the cube
+    # phase-rotated by exp(-j pi m sin theta), through the oracle's C FP64 port; all 128 angles
+    from oracle import c_capon
+    cores, _ = _host()
+    m = np.arange(xn.shape[-1])
+    t0 = time.perf_counter()
+    for sn in sines.astype(np.float32).astype(np.float64):
+        c_capon((xn * np.exp(-1j * np.pi * m * sn)).astype(np.complex64), d, L, Y, want_w=True, nthreads=cores)
+    tc = time.perf_counter() - t0
+    line["cpu_baseline"] = _cpu_line(npow / tc, "powers/s",
+                                     f"all 128 angles x 1020 bins, one FP64 C-port call per rotated cube, {tc:.2f} s")
+    line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)
     return line
 
 
@@ -286,6 +352,18 @@ def adjoint(args):
     line["accuracy"] = {"dphi_err_vs_central_differences_max": float(gerr.max()),
                         "dphi_sample": "8 seeded element phases, error over max |dLoss/dphi|",
                         "P_rel_err_max_64px": float(eP.max())}
+    # CPU leg: the reference has no gradient; the oracle's C FP64 port runs the forward power
+    # of all 8 phase-corrected frames (the reference's own work per frame)
+    cores, _ = _host()
+    with torch.no_grad():
+        xh = (frames * torch.exp(1j * phi)).reshape(F * Nx, Nz, M).cpu().numpy()
+    t0 = time.perf_counter()
+    c_capon(xh, d, L, Y, want_w=False, nthreads=cores)
+    tc = time.perf_counter() - t0
+    line["cpu_baseline"] = _cpu_line(npx / tc, "pixels/s",
+                                     f"forward only (no reference gradient exists): all {npx} pixels of the 8 frames, "
+                                     f"FP64 C port, {tc:.1f} s")
+    line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)
     return line
 
 

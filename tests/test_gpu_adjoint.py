@@ -134,3 +134,68 @@ def test_phase_gradient_differentiable_workload():
         fd = (f(phi0 + e) - f(phi0 - e)) / (2 * h)
         print(f"phi[{m}] fd {fd:.6e} adjoint {g[m]:.6e}", flush=True)
         assert abs(fd - g[m]) <= 1e-3 * np.abs(g).max() + 1e-6
+
+
+def test_differentiable_workload_full_size():
+    """The differentiable workload at its stated size: 8 seeded frames of 128 beams x 512
+    ranges x 64 elements, L = 32, Yavg = 2, per-element phase errors phi (seeded, in
+    [-0.5, 0.5] rad), loss = -mean log P over the 520,192 pixels.  The adjoint's dLoss/dphi,
+    dLoss/d(snapshots) and dLoss/dd against central differences (h = 1e-3) of the FP64
+    forward kernel on 32 seeded coordinates (16 of phi, 16 of the snapshots) and d, each
+    within 1e-3 of the largest gradient entry of its kind (of |dLoss/dd| for d)."""
+    import torch
+    from cos_amd import synth
+    from cos_amd.autograd import capon_power
+    F, Nx, Nz, M, L, Y, d = 8, 128, 512, 64, 32, 2, 0.01
+    say("differentiable workload, full size", F, Nx, Nz, M, L, Y)
+    frames0 = torch.stack([synth.speckle_cube_torch(Nx, Nz, M, seed=50 + f) for f in range(F)])
+    phi0 = torch.from_numpy(synth.config5_phase(M, seed=5)).double().cuda()
+    frames = frames0.clone().requires_grad_(True)
+    phi = phi0.clone().float().requires_grad_(True)
+    dg = torch.tensor(d, device="cuda", requires_grad=True)
+    xs = (frames * torch.exp(1j * phi)).reshape(F * Nx, Nz, M)
+    P = capon_power(xs, dg, L, Y)
+    assert P.numel() == F * Nx * (Nz - 2 * Y) == 520192
+    loss = -torch.log(P).mean()
+    loss.backward()
+    gphi = phi.grad.double().cpu().numpy()
+    gx = frames.grad.cpu().numpy()
+    gd = dg.grad.item()
+    f64 = frames0.to(torch.complex128)
+
+    def f(ph=phi0, dd=d, pert=None):
+        xx = f64 * torch.exp(1j * ph)
+        if pert is not None:
+            xx = xx.clone()
+            xx[pert[0]] += pert[1]
+        xx = xx.to(torch.complex64).reshape(F * Nx, Nz, M).contiguous()
+        return float(-torch.log(fp64_power(xx, dd, L, Y)).mean())
+
+    h = 1e-3
+    rng = np.random.default_rng(6)
+    worst_phi = 0.0
+    for m in rng.choice(M, 16, replace=False):
+        e = torch.zeros(M, dtype=torch.float64, device="cuda")
+        e[m] = h
+        fd = (f(phi0 + e) - f(phi0 - e)) / (2 * h)
+        worst_phi = max(worst_phi, abs(fd - gphi[m]) / np.abs(gphi).max())
+    # snapshot coordinates: dLoss/dRe and dLoss/dIm of x, where x is the frame before the
+    # phase rotation (the chain through exp(j phi) is torch's)
+    worst_x = 0.0
+    scale_x = np.abs(gx).max()
+    for n in range(16):
+        fi, ix, iz, m = int(rng.integers(F)), int(rng.integers(Nx)), int(rng.integers(Y, Nz - Y)), int(rng.integers(M))
+        step = h if n % 2 == 0 else 1j * h
+        rot = torch.exp(1j * phi0[m])
+        fp = f(pert=((fi, ix, iz, m), step * rot))
+        fm = f(pert=((fi, ix, iz, m), -step * rot))
+        fd = (fp - fm) / (2 * h)
+        an = gx[fi, ix, iz, m].real if n % 2 == 0 else gx[fi, ix, iz, m].imag
+        worst_x = max(worst_x, abs(fd - an) / scale_x)
+    fd_d = (f(dd=d * (1 + h)) - f(dd=d * (1 - h))) / (2 * d * h)
+    print(f"loss {loss.item():.6f}; FD worst rel err phi {worst_phi:.2e}, x {worst_x:.2e}; "
+          f"d: fd {fd_d:.6e} adjoint {gd:.6e}", flush=True)
+    assert abs(loss.item() - f()) <= 1e-5 * abs(loss.item())
+    assert worst_phi <= 1e-3
+    assert worst_x <= 1e-3
+    assert abs(fd_d - gd) <= 1e-3 * abs(fd_d)

@@ -993,6 +993,82 @@ def test_beamspace_sonar_slice():
     assert e.max() <= 1e-4
 
 
+def test_beamspace_sonar_full_frame():
+    """The whole sonar frame (512 look directions x 8192 ranges x 96 elements, L = 48,
+    Nb = 5, Yavg = 4; 4,190,208 pixels): every power finite and positive, the frame
+    deterministic across two runs, and a strided sample of 512 pixels (every 8183rd, all
+    beams and depths) against the FP64 oracle within 1e-4."""
+    import torch
+    from cos_amd import CaponPlan, synth
+    Nx, Nz, M, L, Y, Nb, d = 512, 8192, 96, 48, 4, 5, 0.01
+    say("beamspace sonar full frame", Nx, Nz, M, L, Nb, Y)
+    x = synth.config4_torch(Nx, Nz, M, seed=4, device="cuda")
+    plan = CaponPlan()
+    o1 = plan.run_beamspace(x, d, L, Y, Nb, want=("P",))["P"].clone()
+    o2 = plan.run_beamspace(x, d, L, Y, Nb, want=("P",))["P"]
+    torch.cuda.synchronize()
+    npx = Nx * (Nz - 2 * Y)
+    assert o1.numel() == npx == 4190208
+    assert bool(torch.isfinite(o1).all()) and bool((o1 > 0).all())
+    assert torch.equal(o1, o2)
+    sel = np.arange(0, npx, 8183)
+    Nzp = Nz - 2 * Y
+    ib, jb = sel // Nzp, sel % Nzp
+    P = o1.cpu().numpy()[sel]
+    ref = np.empty(len(sel))
+    for b in np.unique(ib):                      # the oracle on each sampled beam's rows
+        k = np.nonzero(ib == b)[0]
+        xb = x[int(b):int(b) + 1].cpu().numpy().astype(np.complex128)
+        ref[k] = O.capon_beamspace(xb, d, L, Y, Nb, pixels=(np.zeros(len(k), int), jb[k]))["P"]
+    e = rel(P, ref)
+    print(f"sonar full frame: P rel err max {e.max():.3e} over {len(sel)} sampled pixels", flush=True)
+    assert e.max() <= 1e-4
+
+
+def test_expanding_aperture_full_frame():
+    """The whole expanding-aperture frame (256 beams x 4096 ranges, M 16..128, L = M/2,
+    5% of rows at d = 1e-5; 1,047,552 pixels) in FP32 with the FP64 fallback, against the
+    all-FP64 run of the same frame on every pixel (P within 1e-4), every weight vector
+    distortionless (sum w = 1, a = 1), the fallback count that of the flagged rows, and a
+    strided sample of 256 pixels against the FP64 oracle."""
+    import torch
+    from cos_amd import CaponPlan, synth
+    Nx, Nz, Mmax, Y = 256, 4096, 128, 2
+    say("expanding aperture full frame", Nx, Nz, Mmax)
+    x, Mz, Lz, off, dz = synth.config3_torch(Nx, Nz, seed=3, Mmax=Mmax, device="cuda")
+    rows = {k: torch.from_numpy(v).cuda() for k, v in dict(L=Lz, M=Mz, off=off, d=dz).items()}
+    plan = CaponPlan()
+    o = plan.run(x, 0.0, 0, Y, rows=rows, precision="fp32+fallback", want=("P", "w", "flags"))
+    o64 = CaponPlan().run(x, 0.0, 0, Y, rows=rows, precision="fp64", want=("P",))
+    torch.cuda.synchronize()
+    Nzp = Nz - 2 * Y
+    npx = Nx * Nzp
+    c = np.arange(Nzp) + Y
+    assert o["P"].numel() == npx == 1047552
+    e = (o["P"].double() - o64["P"].double()).abs() / o64["P"].double()
+    print(f"aperture full frame vs all-FP64: P max {e.max().item():.3e} mean {e.mean().item():.3e}", flush=True)
+    assert e.max().item() <= 1e-4
+    # flagged: the condition bound 1 + L/d above 1e6 in FP32 (the rows at d = 1e-5 with
+    # L > 10; shallower ones stay in FP32)
+    ill = (np.float32(1) + Lz[c].astype(np.float32) / dz[c]) > np.float32(1e6)
+    assert 0 < ill.sum() < (dz[c] < 1e-3).sum()
+    assert plan.fallback_count() == Nx * int(ill.sum())
+    fl = o["flags"].reshape(Nx, Nzp).cpu().numpy()
+    assert ((fl & 1) != 0)[:, ill].all() and (fl[:, ~ill] == 0).all()
+    # distortionless: the first L(z) weights of every pixel sum to 1, the rest are 0
+    w = o["w"].reshape(Nx, Nzp, 64)
+    Lc = torch.from_numpy(Lz[c].astype(np.int64)).cuda()
+    mask = (torch.arange(64, device="cuda")[None, :] < Lc[:, None])[None]
+    s = (w * mask).sum(-1)
+    assert (s - 1).abs().max().item() <= 1e-3
+    sel = np.arange(0, npx, 4099)
+    ref = O.capon_rows(x.cpu().numpy().astype(np.complex128), dz.astype(np.float64), Lz, Mz, off, Y,
+                       pixels=(sel // Nzp, sel % Nzp))
+    eo = rel(o["P"].cpu().numpy()[sel], ref["P"])
+    print(f"aperture full frame vs oracle: P max {eo.max():.3e} over {len(sel)} pixels", flush=True)
+    assert eo.max() <= 1e-4
+
+
 # ------------------------------------------------------------------ reference options
 
 OPTION_CASES = [
