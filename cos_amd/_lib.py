@@ -16,7 +16,7 @@ EXPORTS = (
     "cos_capon_last_error", "cos_capon_plan_create", "cos_capon_plan_destroy", "cos_capon_run",
     "cos_capon_last_fallback_count", "cos_capon_beamspace_run", "cos_capon_backward",
     "cos_capon_subspace_run", "cos_tdib_run", "cos_lca_run", "cos_capon_autotune",
-    "cos_capon_tuning_set", "cos_capon_run_noalloc",
+    "cos_capon_tuning_set", "cos_capon_run_noalloc", "cos_tdib_run_axis",
 )
 
 COS_PREC_FP32 = 0
@@ -83,6 +83,7 @@ def lib() -> C.CDLL:
     L.cos_capon_tuning_set.argtypes = [C.POINTER(CaponDesc), C.POINTER(C.c_int)]
     db = C.c_double
     L.cos_tdib_run.argtypes = [vp, ip, ip, db, db, vp, vp, db, db, vp, vp, ip, ip, fl, vp, vp, vp]
+    L.cos_tdib_run_axis.argtypes = [vp, ip, ip, vp, vp, vp, db, db, vp, vp, ip, ip, fl, vp, vp, vp]
     L.cos_lca_run.argtypes = [vp, ip, ip, ip, vp, ip, ip, ip, vp, vp, vp]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int

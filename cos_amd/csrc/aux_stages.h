@@ -8,7 +8,8 @@ namespace cosk {
 struct TdibLaunch {
   const float2* mf;        // [n_h][n_t] matched-filtered channel data
   int n_h, n_t;
-  double t0, dt;           // uniform time axis t0 + i dt
+  double t0, dt;           // uniform time axis t0 + i dt (when taxis is null)
+  const double* taxis;     // optional [n_t] strictly increasing time axis
   const double* x_r;       // [n_h] hydrophone along-track positions
   const double* D;         // [n_h] hydrophone depths
   double inv_c, fc;        // 1/sound speed, carrier frequency

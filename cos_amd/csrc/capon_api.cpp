@@ -738,21 +738,26 @@ int cos_capon_subspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, con
   return 0;
 }
 
-int cos_tdib_run(const cos_cf32* mfdata, int n_h, int n_t, double t0, double dt, const double* x_r,
-                 const double* D, double c, double fc, const double* xarr, const double* yarr,
-                 int n_x, int n_y, float fill, cos_cf32* cube, cos_cf32* image, void* stream) {
+namespace {
+
+int tdib_common(const cos_cf32* mfdata, int n_h, int n_t, double t0, double dt, const double* taxis,
+                const double* x_r, const double* D, double c, double fc, const double* xarr, const double* yarr,
+                int n_x, int n_y, float fill, cos_cf32* cube, cos_cf32* image, void* stream) {
   g_last_error.clear();
   if (!mfdata || !x_r || !D || !xarr || !yarr) return fail(hipErrorInvalidValue, "NULL input");
   if (n_h < 1 || n_t < 1 || n_x < 1 || n_y < 1) return fail(hipErrorInvalidValue, "sizes must be >= 1");
-  if (!(dt > 0.0) || !(c > 0.0) || !std::isfinite(t0) || !std::isfinite(fc))
-    return fail(hipErrorInvalidValue, "need dt > 0, c > 0, finite t0 and fc");
+  if (!taxis && (!(dt > 0.0) || !std::isfinite(t0)))
+    return fail(hipErrorInvalidValue, "need dt > 0 and a finite t0");
+  if (!(c > 0.0) || !std::isfinite(fc)) return fail(hipErrorInvalidValue, "need c > 0 and a finite fc");
   if ((long)n_x * n_y > (1L << 31) - 1) return fail(hipErrorInvalidValue, "too many pixels for one call");
   cosk::TdibLaunch a;
+  std::memset(&a, 0, sizeof(a));
   a.mf = (const float2*)mfdata;
   a.n_h = n_h;
   a.n_t = n_t;
   a.t0 = t0;
-  a.dt = dt;
+  a.dt = taxis ? 1.0 : dt;
+  a.taxis = taxis;
   a.x_r = x_r;
   a.D = D;
   a.inv_c = 1.0 / c;
@@ -766,6 +771,26 @@ int cos_tdib_run(const cos_cf32* mfdata, int n_h, int n_t, double t0, double dt,
   a.image = (float2*)image;
   hipError_t e = cosk::launch_tdib(a, (hipStream_t)stream);
   return e == hipSuccess ? 0 : fail_hip(e, "tdib kernel");
+}
+
+}  // namespace
+
+int cos_tdib_run(const cos_cf32* mfdata, int n_h, int n_t, double t0, double dt, const double* x_r,
+                 const double* D, double c, double fc, const double* xarr, const double* yarr,
+                 int n_x, int n_y, float fill, cos_cf32* cube, cos_cf32* image, void* stream) {
+  return tdib_common(mfdata, n_h, n_t, t0, dt, nullptr, x_r, D, c, fc, xarr, yarr, n_x, n_y, fill, cube, image,
+                     stream);
+}
+
+int cos_tdib_run_axis(const cos_cf32* mfdata, int n_h, int n_t, const double* taxis, const double* x_r,
+                      const double* D, double c, double fc, const double* xarr, const double* yarr,
+                      int n_x, int n_y, float fill, cos_cf32* cube, cos_cf32* image, void* stream) {
+  if (!taxis) {
+    g_last_error = "taxis is NULL";
+    return hipErrorInvalidValue;
+  }
+  return tdib_common(mfdata, n_h, n_t, 0.0, 0.0, taxis, x_r, D, c, fc, xarr, yarr, n_x, n_y, fill, cube, image,
+                     stream);
 }
 
 int cos_lca_run(const cos_cf32* Z, int Nx, int Ny, int Nw, const float* wrf, int wy, int wx, int mode,

@@ -8,7 +8,9 @@
 //   t   = (r_t + r_r) / c
 //   v_n = interp(mfdata[n], t) * exp(j 2 pi fc t)                  linear interpolation on
 //         the time axis; outside the axis the value is the fill (the reference's
-//         interp1d(bounds_error=False) gives NaN)
+//         interp1d(bounds_error=False) gives NaN).  On a general (strictly increasing)
+//         axis the interval is scipy's: hi = the first index with taxis[hi] >= t
+//         (searchsorted, left), clipped to [1, n_t - 1], lo = hi - 1, by binary search.
 //   image[ny][nx] = (1/n_h) sum_n v_n
 // The delayed channels v_n are also written as the cube [n_x][n_y][n_h], which is the
 // layout the Capon kernels read ([beam][range][element]).
@@ -62,13 +64,36 @@ __global__ __launch_bounds__(256) void tdib_kernel(const TdibLaunch a, const int
       const double dx = ex - a.x_r[n], dz = zz - a.D[n];
       const double r_r = sqrt(dx * dx + ya2 + dz * dz);
       const double t = (r_t + r_r) * a.inv_c;
-      // linear interpolation on the uniform axis t0 + i dt, i < n_t
-      const double u = (t - a.t0) * inv_dt;
+      // linear interpolation: interval i .. i+1 and weight f of t in it
       float2 v = make_float2(a.fill, a.fill);
-      if (u >= 0.0 && u <= (double)(a.n_t - 1)) {
-        int i = (int)u;                          // 0 <= i <= n_t - 1
-        if (i > a.n_t - 2) i = a.n_t > 1 ? a.n_t - 2 : 0;
-        const float f = (float)(u - (double)i);  // 1 at the last sample
+      int i = 0;
+      float f = 0.f;
+      bool in = false;
+      if (a.taxis) {
+        const double* ta = a.taxis;
+        in = t >= ta[0] && t <= ta[a.n_t - 1];
+        if (in && a.n_t > 1) {
+          int lo = 0, hi = a.n_t - 1;            // ta[lo] <= t <= ta[hi]: first index >= t
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (ta[mid] < t) lo = mid;
+            else hi = mid;
+          }
+          if (ta[lo] >= t) hi = lo;              // t == ta[0]
+          hi = hi < 1 ? 1 : hi;
+          i = hi - 1;
+          f = (float)((t - ta[i]) / (ta[hi] - ta[i]));
+        }
+      } else {
+        const double u = (t - a.t0) * inv_dt;
+        in = u >= 0.0 && u <= (double)(a.n_t - 1);
+        if (in) {
+          i = (int)u;                            // 0 <= i <= n_t - 1
+          if (i > a.n_t - 2) i = a.n_t > 1 ? a.n_t - 2 : 0;
+          f = (float)(u - (double)i);            // 1 at the last sample
+        }
+      }
+      if (in) {
         const float2 s0 = a.mf[(size_t)n * a.n_t + i];
         const float2 s1 = a.mf[(size_t)n * a.n_t + (a.n_t > 1 ? i + 1 : i)];
         const float re = fmaf(f, s1.x - s0.x, s0.x);

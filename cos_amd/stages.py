@@ -22,26 +22,42 @@ def _dev(a, dtype):
     return t.contiguous()
 
 
-def _uniform_axis(mtaxe):
+def _axis(mtaxe):
+    """(t0, dt, None) for a uniform axis (cos_tdib_run), else (0, 0, device float64 axis) for
+    cos_tdib_run_axis; the axis must increase strictly."""
+    import torch
     t = np.asarray(mtaxe.detach().cpu() if hasattr(mtaxe, "detach") else mtaxe, dtype=np.float64).ravel()
     if t.size == 1:
-        return float(t[0]), 1.0
+        return float(t[0]), 1.0, None
+    if not np.all(np.diff(t) > 0):
+        raise ValueError("mtaxe must increase strictly")
     dt = (t[-1] - t[0]) / (t.size - 1)
-    if not dt > 0 or np.max(np.abs(np.diff(t) - dt)) > 1e-6 * dt:
-        raise ValueError("mtaxe must be a uniform, increasing time axis")
-    return float(t[0]), float(dt)
+    if np.max(np.abs(np.diff(t) - dt)) <= 1e-9 * dt:
+        return float(t[0]), float(dt), None
+    return 0.0, 0.0, torch.as_tensor(t, dtype=torch.float64, device="cuda")
+
+
+def _run_tdib(mf, n_h, n_t, mtaxe, xr, dd, c, fc, xa, ya, n_x, n_y, fill, cube, image, stream):
+    t0, dt, ax = _axis(mtaxe)
+    args = (xr.data_ptr(), dd.data_ptr(), float(c), float(fc), xa.data_ptr(), ya.data_ptr(), n_x, n_y, float(fill),
+            cube.data_ptr() if cube is not None else None, image.data_ptr() if image is not None else None, stream)
+    if ax is None:
+        check(lib().cos_tdib_run(mf.data_ptr(), n_h, n_t, t0, dt, *args))
+    else:
+        if ax.numel() != n_t:
+            raise ValueError("mtaxe must have n_t samples")
+        check(lib().cos_tdib_run_axis(mf.data_ptr(), n_h, n_t, ax.data_ptr(), *args))
 
 
 def tdib_cube(mfdata, mtaxe, x_r, D, c, fc, xarr, yarr, *, fill=float("nan"), want_image=True, stream=None):
     """Delay stage: returns (cube [n_x][n_y][n_h] complex64, image [n_y][n_x] complex64 or None).
-    mfdata [n_h][n_t]; mtaxe [n_t] uniform; x_r, D [n_h]; xarr, yarr [n_x][n_y] (xarr may be
-    [n_x], broadcast along the beam as in the reference)."""
+    mfdata [n_h][n_t]; mtaxe [n_t] strictly increasing (uniform or not); x_r, D [n_h];
+    xarr, yarr [n_x][n_y] (xarr may be [n_x], broadcast along the beam as in the reference)."""
     import torch
     mf = _dev(mfdata, torch.complex64)
     if mf.dim() != 2:
         raise ValueError("mfdata must be [n_h][n_t] (several banks are not supported)")
     n_h, n_t = mf.shape
-    t0, dt = _uniform_axis(mtaxe)
     ya = _dev(yarr, torch.float64)
     xa = _dev(xarr, torch.float64)
     if xa.dim() == 1:
@@ -53,9 +69,7 @@ def tdib_cube(mfdata, mtaxe, x_r, D, c, fc, xarr, yarr, *, fill=float("nan"), wa
     image = torch.empty((n_y, n_x), dtype=torch.complex64, device="cuda") if want_image else None
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
-    check(lib().cos_tdib_run(mf.data_ptr(), n_h, n_t, t0, dt, xr.data_ptr(), dd.data_ptr(), float(c), float(fc),
-                             xa.data_ptr(), ya.data_ptr(), n_x, n_y, float(fill), cube.data_ptr(),
-                             image.data_ptr() if image is not None else None, stream))
+    _run_tdib(mf, n_h, n_t, mtaxe, xr, dd, c, fc, xa, ya, n_x, n_y, fill, cube, image, stream)
     return cube, image
 
 
@@ -64,7 +78,6 @@ def tdib(mfdata, mtaxe, x_r, D, c, fc, xarr, yarr, *, fill=float("nan"), stream=
     import torch
     mf = _dev(mfdata, torch.complex64)
     n_h, n_t = mf.shape
-    t0, dt = _uniform_axis(mtaxe)
     ya = _dev(yarr, torch.float64)
     xa = _dev(xarr, torch.float64)
     if xa.dim() == 1:
@@ -75,8 +88,7 @@ def tdib(mfdata, mtaxe, x_r, D, c, fc, xarr, yarr, *, fill=float("nan"), stream=
     image = torch.empty((n_y, n_x), dtype=torch.complex64, device="cuda")
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
-    check(lib().cos_tdib_run(mf.data_ptr(), n_h, n_t, t0, dt, xr.data_ptr(), dd.data_ptr(), float(c), float(fc),
-                             xa.data_ptr(), ya.data_ptr(), n_x, n_y, float(fill), None, image.data_ptr(), stream))
+    _run_tdib(mf, n_h, n_t, mtaxe, xr, dd, c, fc, xa, ya, n_x, n_y, fill, None, image, stream)
     return image
 
 

@@ -209,8 +209,8 @@ int cos_capon_backward(cos_capon_plan* plan, const cos_capon_desc* desc, const c
  *
  * Delay stage: time-domain interpolation beamformer, replacing
  * framework/beamformer/delay/tdibC.pyx:9-67 (tdib(mfdata, mtaxe, x_r, D, c, fc, xarr, yarr)).
- * mfdata [n_h][n_t] matched-filtered channels on the uniform axis t0 + i*dt (the reference
- * takes any sorted axis; this entry takes a uniform one); x_r, D [n_h] hydrophone positions;
+ * mfdata [n_h][n_t] matched-filtered channels on the uniform axis t0 + i*dt (any strictly
+ * increasing axis: cos_tdib_run_axis); x_r, D [n_h] hydrophone positions;
  * xarr, yarr [n_x][n_y] pixel coordinates; all device pointers.  Writes the delayed,
  * carrier-mixed channels cube [n_x][n_y][n_h] (the Capon input layout) and/or the
  * delay-and-sum image [n_y][n_x] = (1/n_h) sum over hydrophones (the reference's return
@@ -218,6 +218,14 @@ int cos_capon_backward(cos_capon_plan* plan, const cos_capon_desc* desc, const c
 int cos_tdib_run(const cos_cf32* mfdata, int n_h, int n_t, double t0, double dt, const double* x_r,
                  const double* D, double c, double fc, const double* xarr, const double* yarr,
                  int n_x, int n_y, float fill, cos_cf32* cube, cos_cf32* image, void* stream);
+
+/* As cos_tdib_run on any strictly increasing time axis taxis [n_t] (device, seconds), as
+ * the reference's interp1d takes (tdibC.pyx:55-56): the interval of t is scipy's, the first
+ * sample at or after t (searchsorted, left side) clipped to 1 .. n_t-1 and the one before it,
+ * found by binary search; times outside [taxis[0], taxis[n_t-1]] give `fill`. */
+int cos_tdib_run_axis(const cos_cf32* mfdata, int n_h, int n_t, const double* taxis, const double* x_r,
+                      const double* D, double c, double fc, const double* xarr, const double* yarr,
+                      int n_x, int n_y, float fill, cos_cf32* cube, cos_cf32* image, void* stream);
 
 /* Window selection: low-complexity adaptive beamformer, replacing
  * framework/beamformer/lca/lcacx.pyx:27-217 (lca(Z, wrf)).  Z [Nx][Ny][Nw] images formed

@@ -53,6 +53,31 @@ def test_tdib_cube_and_image_match_oracle(n_h, n_x, n_y):
     np.testing.assert_array_equal(np.isnan(image.real), np.isnan(iref.real))
 
 
+@pytest.mark.parametrize("n_t", [6000, 2])
+def test_tdib_general_time_axis(n_t):
+    """A strictly increasing but non-uniform time axis (jittered steps, as interp1d takes):
+    cos_tdib_run_axis against the oracle (np.interp, pinned against scipy's interp1d) within
+    2e-6 of the data scale, NaN exactly outside the axis; and a two-sample axis."""
+    from cos_amd import stages
+    say("tdib general axis", n_t)
+    mf, _, x_r, D, xarr, yarr = _scene(3, n_h=16, n_x=4, n_y=200, n_t=max(n_t, 2))
+    rng = np.random.default_rng(30)
+    if n_t == 2:
+        mtaxe = np.array([0.02, 0.08])
+    else:
+        mtaxe = 0.01 + np.cumsum(rng.uniform(0.2, 1.8, n_t)) * 2.0e-5
+    c, fc = 1500.0, 1.0e5
+    cube, image = stages.tdib_cube(mf, mtaxe, x_r, D, c, fc, xarr, yarr)
+    cube = cube.cpu().numpy()
+    ref = S.tdib_cube(mf.astype(np.complex128), mtaxe, x_r, D, c, fc, xarr, yarr)
+    np.testing.assert_array_equal(np.isnan(cube.real), np.isnan(ref.real))
+    ok = ~np.isnan(ref)
+    assert ok.mean() > 0.3
+    err = np.abs(cube[ok] - ref[ok]).max() / np.abs(mf).max()
+    print(f"tdib general-axis cube err {err:.2e}", flush=True)
+    assert err <= 2e-6
+
+
 def test_tdib_feeds_capon():
     """The delayed cube (fill 0 outside the axis) is the Capon input layout: run the FP64
     Capon kernel on it and compare with the oracle on the same cube."""

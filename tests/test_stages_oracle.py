@@ -20,7 +20,14 @@ def test_tdib_interpolation_matches_scipy_interp1d():
     rng = np.random.default_rng(1)
     c, fc = 1500.0, 1.0e5
     x_r, D, xarr, yarr = _geometry(rng)
-    mtaxe = np.linspace(0.02, 0.09, 900)
+    uniform = np.linspace(0.02, 0.09, 900)
+    jittered = 0.02 + np.cumsum(rng.uniform(0.3, 1.7, 900)) * (0.07 / 900)   # any sorted axis
+    for mtaxe in (uniform, jittered):
+        _check_interp1d(rng, c, fc, x_r, D, xarr, yarr, mtaxe)
+
+
+def _check_interp1d(rng, c, fc, x_r, D, xarr, yarr, mtaxe):
+    from scipy import interpolate
     mf = rng.standard_normal((8, 900)) + 1j * rng.standard_normal((8, 900))
     cube = S.tdib_cube(mf, mtaxe, x_r, D, c, fc, xarr, yarr)
     xx, zz = np.mean(x_r) / 2, np.mean(D) / 2
