@@ -163,7 +163,7 @@ int check_args(float d, int L, int Yavg, int M, int Nx, int Nz, int Nb) {
   if (Nb > L) return fail(hipErrorInvalidValue, "Nb can not be larger than L");
   if (L > M) return fail(hipErrorInvalidValue, "L > M");
   if (2 * Yavg >= Nz) return fail(hipErrorInvalidValue, "2*Yavg >= Nz");
-  if (L > 64) return fail(hipErrorInvalidValue, "L > 64 is not supported");
+  if (L > 72) return fail(hipErrorInvalidValue, "L > 72 is not supported (the reference solver's limit)");
   if (!(d >= 0.0f) || !std::isfinite(d)) return fail(hipErrorInvalidValue, "d must be finite and >= 0");
   if ((long)Nx * Nz * M > (1L << 31) - 1) return fail(hipErrorInvalidValue, "cube too large for one call");
   return 0;
@@ -856,6 +856,7 @@ int cos_capon_beamspace_run(cos_capon_plan* plan, const cos_capon_desc* desc, in
     return fail(hipErrorInvalidValue, "beamspace path takes the default options");
   if (int e = check_args(ds.d, ds.L, ds.Yavg, ds.M, ds.Nx, ds.Nz, Nb)) return e;
   if (Nb < 1 || Nb > 8) return fail(hipErrorInvalidValue, "beamspace path supports 1 <= Nb <= 8");
+  if (ds.L > 64) return fail(hipErrorInvalidValue, "beamspace path supports L <= 64");
   if (ds.off < 0 || ds.off + ds.M > ds.Mstride) return fail(hipErrorInvalidValue, "off + M > Mstride");
   if (2 * ds.Yavg > 120) return fail(hipErrorInvalidValue, "beamspace path supports Yavg <= 60");
   // the kernel addresses a wave's 64 rows with 32-bit buffer offsets below 2^31
@@ -955,7 +956,17 @@ int cos_capon_get_capon_p(cos_cf32* z, cos_cf32* w, cos_cf32* R, float* P, const
   std::memset(&ds, 0, sizeof(ds));
   ds.Nx = Nx; ds.Nz = Nz; ds.Mstride = M; ds.M = M; ds.L = L; ds.off = 0; ds.Yavg = Yavg; ds.d = d;
   ds.precision = COS_PREC_FP32;
-  if (Nb == 0) {
+  if (Nb == 0 && L > 64) {
+    // subarrays past the fused kernel's lane shapes: one block per pixel in LDS
+    cosk::CaponLaunch a = base_launch(ds, pl->x.p);
+    a.P = P ? pl->P.as<float>() : nullptr;
+    a.z = z ? pl->z.as<float2>() : nullptr;
+    a.w = w ? pl->w.as<float2>() : nullptr;
+    a.wstride = L;
+    a.R = R ? pl->R.as<float2>() : nullptr;
+    a.Rstride = L * L;
+    if ((e = cosk::launch_capon_wide(a, nullptr)) != hipSuccess) return fail_hip(e, "wide kernel");
+  } else if (Nb == 0) {
     int r = cos_capon_run(pl, &ds, pl->x.as<cos_cf32>(), P ? pl->P.as<float>() : nullptr,
                           z ? pl->z.as<cos_cf32>() : nullptr, w ? pl->w.as<cos_cf32>() : nullptr, L,
                           R ? pl->R.as<cos_cf32>() : nullptr, nullptr, nullptr);
@@ -1004,7 +1015,7 @@ int cos_buildr_get_r(cos_cf32* R, const cos_cf32* x, float d, int L, int Yavg, i
   if (L > M) return fail(hipErrorInvalidValue, "L > M");
   if (2 * Yavg >= Nz) return fail(hipErrorInvalidValue, "Nz <= Yavg");
   if (!R || !x) return fail(hipErrorInvalidValue, "R and x must be non-NULL");
-  if (Nb == 0 && L > 64) return fail(hipErrorInvalidValue, "L > 64 is not supported");
+  if (Nb == 0 && L > 128) return fail(hipErrorInvalidValue, "L > 128 is not supported (the reference buildR's limit)");
   std::lock_guard<std::mutex> lk(g_default_mu);
   cos_capon_plan* pl = default_plan();
   const int K = M - L + 1;

@@ -40,8 +40,9 @@ hipError_t launch_capon_f32_shape(const CaponLaunch& a, int shape, int Mmax, hip
 
 hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why) {
   if (Lmax <= 64) return launch_capon_f32_shape(a, default_shape(Lmax), Mmax, s, why);
-  if (why) *why = "L > 64 is not supported by the fused kernel";
-  return hipErrorInvalidValue;
+  const hipError_t e = launch_capon_wide(a, s);
+  if (e == hipErrorInvalidValue && why) *why = "subarray length beyond the wide kernel (L <= 72; 128 for R only)";
+  return e;
 }
 
 static hipError_t f64_launch(const CaponLaunch& a, int Lmax, int Mmax, bool list, hipStream_t s, const char** why) {

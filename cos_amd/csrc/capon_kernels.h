@@ -91,5 +91,8 @@ hipError_t launch_capon_f64(const CaponLaunch& a, int Lmax, int Mmax, hipStream_
 hipError_t launch_capon_f64_list(const CaponLaunch& a, int Lmax, int Mmax, bool persist, hipStream_t s,
                                  const char** why);
 hipError_t launch_capon_bwd(const CaponLaunch& a, int Lmax, int Mmax, hipStream_t s, const char** why);
+// Uniform L in (64, 72] (or up to 128 with cov_only): one block per pixel in LDS (wide.hip);
+// the default options only (no tables, lists, steering).
+hipError_t launch_capon_wide(const CaponLaunch& a, hipStream_t s);
 
 }  // namespace cosk

@@ -35,7 +35,8 @@ typedef struct { float re, im; } cos_cf32;
  * R may be NULL to skip it.  d is the relative diagonal loading (d*trace/L).
  * Nb > 0 selects the reference's default matrix beamspace path with Nb Butler beams
  * (Capon.cpp:285-402).  Requirements as the reference: 1 <= L <= M, Yavg >= 0,
- * 2*Yavg < Nz, 0 <= Nb <= L; in addition L <= 64. */
+ * 2*Yavg < Nz, 0 <= Nb <= L, L <= 72 (the solver's limit, Solver.cpp:34-42); for
+ * 64 < L <= 72 (Nb = 0) a kernel with one block per pixel in LDS serves the call. */
 int cos_capon_get_capon(cos_cf32* z, cos_cf32* w, cos_cf32* R, const cos_cf32* x, float d,
                         int L, int Yavg, int M, int Nx, int Nz, int Nb);
 
@@ -49,7 +50,7 @@ int cos_capon_get_capon_p(cos_cf32* z, cos_cf32* w, cos_cf32* R, float* P, const
  * Nb == 0: R [Nx][Nz-2Y][L][L] loaded unnormalised conj-stored covariance of x [Nx][Nz][M].
  * Nb > 0: x holds beamspace-transformed subarrays [Nx][Nz][K][Nb] (K = M-L+1) and R is
  * [Nx][Nz-2Y][Nb][Nb] with loading d*trace/Nb (BuildR.cpp:82-86, buildR_kernel.cu:169-251).
- * R_on_gpu / x_on_gpu: nonzero when the pointer is device memory. */
+ * R_on_gpu / x_on_gpu: nonzero when the pointer is device memory.  L <= 128 (Nb = 0). */
 int cos_buildr_get_r(cos_cf32* R, const cos_cf32* x, float d, int L, int Yavg, int M, int Nx,
                      int Nz, int Nb, int R_on_gpu, int x_on_gpu);
 

@@ -38,7 +38,7 @@ def _err(code):
     (dict(Nb=9), "Nb can not be larger than L"),
     (dict(L=9, M=8), "L > M"),
     (dict(Yavg=2, Nz=4), "2*Yavg >= Nz"),
-    (dict(L=65, M=70), "L > 64"),
+    (dict(L=73, M=80), "L > 72"),
 ])
 def test_get_capon_argument_errors(args, msg):
     p = dict(d=0.1, L=4, Yavg=1, M=8, Nx=2, Nz=6, Nb=0)

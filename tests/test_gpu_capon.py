@@ -57,6 +57,60 @@ def test_dropin_matches_oracle(M, L, Y, Nx, Nz, d):
     assert np.max(np.abs(R - ref["R"])) <= 2e-6 * np.max(np.abs(ref["R"])) * np.sqrt(M)
 
 
+WIDE_CASES = [
+    # (M, L, Y, Nx, Nz, d): subarrays past the fused kernel's 64 lanes, up to the reference
+    # solver's 72 (RTC/Solver.cpp:34-42)
+    (80, 65, 1, 2, 12, 0.2), (100, 70, 2, 2, 11, 0.05), (144, 72, 1, 3, 8, 0.01), (72, 72, 0, 2, 5, 0.2),
+]
+
+
+@pytest.mark.parametrize("M,L,Y,Nx,Nz,d", WIDE_CASES)
+def test_dropin_wide_subarrays_match_oracle(M, L, Y, Nx, Nz, d):
+    from cos_amd import getCaponCUDAPy, get_capon
+    say("dropin wide", M, L, Y, Nx, Nz, d)
+    x = uniform_cube(300 + M + L, Nx, Nz, M)
+    z, R, w, e = getCaponCUDAPy(x, d, L, Y, np.array([0]), False, False)
+    assert e == 0
+    ref = O.reference_gpu_outputs(x.astype(np.complex128), float(np.float32(d)), L, Y)
+    assert np.max(np.abs(z - ref["z"])) <= 2e-4 * np.max(np.abs(ref["z"])) + 1e-6
+    assert np.max(np.abs(w - ref["w"])) <= 2e-4 * np.max(np.abs(ref["w"]))
+    assert np.max(np.abs(R - ref["R"])) <= 2e-6 * np.max(np.abs(ref["R"])) * np.sqrt(M)
+    P = get_capon(x, d, L, Y)["P"].ravel()
+    Pref = O.capon(x.astype(np.complex128), float(np.float32(d)), L, Y)["P"].ravel()
+    assert rel(P, Pref).max() <= 1e-4
+
+
+def test_dropin_wide_beamspace_and_limits():
+    """L = 72 with Nb = 8 beams (the element-space S_L from the wide kernel, then B S_L Bᴴ),
+    and L = 73 rejected with the solver's limit in the message."""
+    from cos_amd import getCaponCUDAPy
+    from cos_amd._lib import lib
+    say("dropin wide beamspace", 80, 72, 8)
+    x = uniform_cube(77, 2, 8, 80)
+    z, R, w, e = getCaponCUDAPy(x, 0.2, 72, 1, np.zeros((8, 72)), False, False)
+    assert e == 0
+    ref = O.capon_beamspace_matrix(x.astype(np.complex128), float(np.float32(0.2)), 72, 1, 8)
+    assert np.abs(z.ravel() - ref["z"]).max() <= 5e-4 * np.abs(ref["z"]).max()
+    assert np.abs(w.reshape(-1, 72) - ref["w"]).max() <= 5e-4 * np.abs(ref["w"]).max()
+    h = np.zeros(80 * 8 * 2, np.complex64)
+    assert lib().cos_capon_get_capon_p(None, None, None, None, h.ctypes.data, 0.1, 73, 0, 80, 2, 4, 0) != 0
+    assert b"L > 72" in lib().cos_capon_last_error()
+
+
+@pytest.mark.parametrize("M,L", [(100, 100), (140, 128), (90, 65)])
+def test_buildr_wide_subarrays(M, L):
+    """BuildR::getR up to the reference's L = 128 (RTC/buildR_kernel.cu:260-264)."""
+    from cos_amd import BuildR
+    say("buildR wide", M, L)
+    x = uniform_cube(9 + L, 2, 6, M)
+    Nx, Nz, M = x.shape
+    Y, d = 1, 0.1
+    R = np.zeros((Nx, Nz - 2 * Y, L, L), np.complex64)
+    assert BuildR().getR(R, x, d, L, Y, M, Nx, Nz, 0) == 0
+    ref = O.reference_gpu_outputs(x.astype(np.complex128), float(np.float32(d)), L, Y)["R"]
+    assert np.abs(R - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
 def test_dropin_power_extension():
     from cos_amd import get_capon
     say("dropin power", 32, 16, 1)
