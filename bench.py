@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="frame",
-                    choices=["frame", "aperture", "beamspace", "steering", "adjoint", "tdib", "lca", "shard"],
+                    choices=["frame", "aperture", "beamspace", "steering", "adjoint", "tdib", "lca", "solver", "shard"],
                     help="frame: the headline 256x2048x64 L=32 frame (default); the others time the "
                          "expanding-aperture, sonar beamspace, steered and differentiable workloads")
     args = ap.parse_args()

@@ -746,6 +746,32 @@ def test_solver_component(N):
     np.testing.assert_allclose(x1[0], ref[0], rtol=0, atol=1e-4 * np.abs(ref[0]).max())
 
 
+@pytest.mark.parametrize("N,batch", [(2, 3000), (17, 640), (64, 100), (72, 200)])
+def test_solver_matches_oracle_with_exchanges(N, batch):
+    """Systems whose diagonal is small against the rest of their column, so that most steps
+    exchange rows, against the oracle's restatement of zfsolve_batch (pivot |re| + |im|),
+    over a batch of many waves."""
+    from cos_amd import Solver
+    from oracle import solver_oracle as SO
+    say("solver exchanges", N, batch)
+    rng = np.random.default_rng(40 + N)
+    A = rng.standard_normal((batch, N, N)) + 1j * rng.standard_normal((batch, N, N))
+    A[:, np.arange(N), np.arange(N)] *= 1e-3
+    A = A.astype(np.complex64)
+    b = (rng.standard_normal((batch, N)) + 1j * rng.standard_normal((batch, N))).astype(np.complex64)
+    Acm = np.ascontiguousarray(np.swapaxes(A, 1, 2))
+    x = np.zeros((batch, N), np.complex64)
+    assert Solver().solve(x, Acm, b, N, batch) == 0
+    ref, piv = SO.solve_batch(Acm.reshape(batch, -1), b, return_pivots=True)
+    assert (piv != np.arange(N)).mean() > 0.5 or N == 2
+    # residual relative to |A| |x| + |b|, the backward error of a pivoted elimination
+    r = np.abs(np.einsum("sij,sj->si", A.astype(complex), x.astype(complex)) - b)
+    scale = np.einsum("sij,sj->si", np.abs(A), np.abs(x)) + np.abs(b)
+    assert (r / scale).max() <= 2e-5 * N
+    err = np.abs(x - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    assert np.median(err) <= 1e-4
+
+
 @pytest.mark.parametrize("R_on_gpu,x_on_gpu", [(True, True), (True, False), (False, True)])
 def test_buildr_device_pointers(R_on_gpu, x_on_gpu):
     """BuildR::getR with device buffers (RTC/BuildR.cpp:59-103): R and/or x as device
