@@ -210,6 +210,12 @@ def main():
     ref_ms = timed(lambda: step(refine=True, graph=False), max(3, args.steps // 4))
     res["refined"] = {"ms_per_frame": round(ref_ms, 3), "pixels_per_s": NPX / (ref_ms * 1e-3)}
     res["refined"].update(errors(out["P"], out["w"]))
+    # the displacement recursion entirely on the VALU (the default forms its rank-two
+    # corrections on the FP32 matrix cores)
+    vr_ms = timed(lambda: step(cov_mfma="valu", graph=False), max(3, args.steps // 4))
+    res["valu_recursion"] = {"ms_per_frame": round(vr_ms, 3), "pixels_per_s": NPX / (vr_ms * 1e-3),
+                             "path": "covariance recursion with its rank-two corrections on the VALU"}
+    res["valu_recursion"].update(errors(out["P"], out["w"]))
     # the covariance on the matrix cores (BF16 hi/lo split, FP32 accumulation) into a plan
     # buffer, then the FP32 solve from it
     mf_ms = timed(lambda: step(cov_mfma=True, graph=False), max(3, args.steps // 4))
@@ -235,7 +241,7 @@ def main():
                    "fallback_pixels": nfb},
         "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-                     "traffic": measured_traffic(), "kernel": "capon_kernel<float,16,2,5,false,false,0>",
+                     "traffic": measured_traffic(), "kernel": "capon_kernel<float,16,2,5,false,false,0,false,2>",
                      "kernel_ms": round(kern_ms, 4), "flop_per_pixel": FLOP_PER_PX},
         "accuracy": acc,
         "wall_s_timed": round(wall, 3),

@@ -159,13 +159,17 @@ class CaponPlan:
         True: covariance on the matrix cores (BF16 hi/lo split) into R, then the FP32 solve
         from it (L <= 64, row tables allowed, default options); "fused": per-snapshot Grams
         on the matrix cores inside the fused kernel where it applies (16 < L <= 32, windows
-        of up to five rows, default options).  Returns a dict of output tensors.
+        of up to five rows, default options); "rank": the recursion's rank-two corrections
+        summed per window on the FP32 matrix cores inside the fused kernel (same reach; what
+        False does by default); "valu": the recursion entirely on the VALU.  Returns a dict
+        of output tensors.
         """
         ds, o, ptr, wstride, stream = self._prepare(
             x, d, L, Yavg, M=M, off=off, precision=precision, want=want, rows=rows, steer_sin=steer_sin,
             cond_thresh=cond_thresh, out=out, stream=stream, graph=graph, fb_avg=fb_avg, z_mode=z_mode,
             p_mode=p_mode, refine=refine)
-        ds.cov_mfma = 2 if cov_mfma in ("fused", 2) else int(bool(cov_mfma))
+        ds.cov_mfma = {"fused": 2, "rank": 3, "valu": 4}.get(cov_mfma, cov_mfma) if cov_mfma not in (True, False) \
+            else int(cov_mfma)
         check(lib().cos_capon_run(self._h, C.byref(ds), x.data_ptr(), ptr("P"), ptr("z"), ptr("w"),
                                   wstride, ptr("R"), ptr("flags"), stream))
         return o

@@ -311,11 +311,11 @@ int validate(const cos_capon_desc& ds, cos_cf32* R, cos_cf32* w, int wstride) {
     return fail(hipErrorInvalidValue, "steering takes the default z and P definitions");
   if (ds.refine && (ds.precision == COS_PREC_FP64 || ds.fb_avg))
     return fail(hipErrorInvalidValue, "refinement applies to the FP32 paths without forward-backward averaging");
-  if (ds.cov_mfma < 0 || ds.cov_mfma > 2) return fail(hipErrorInvalidValue, "bad cov_mfma");
+  if (ds.cov_mfma < 0 || ds.cov_mfma > 4) return fail(hipErrorInvalidValue, "bad cov_mfma");
   if (ds.cov_mfma == 1 && (ds.precision == COS_PREC_FP64 || ds.nsteer > 0 || ds.fb_avg || ds.z_mode ||
                            ds.p_mode || ds.refine))
     return fail(hipErrorInvalidValue, "the MFMA covariance takes an FP32 path and the default options");
-  if (ds.cov_mfma == 2 && ds.precision == COS_PREC_FP64)
+  if ((ds.cov_mfma == 2 || ds.cov_mfma == 3) && ds.precision == COS_PREC_FP64)
     return fail(hipErrorInvalidValue, "the fused MFMA covariance takes an FP32 path");
   return 0;
 }
@@ -346,7 +346,7 @@ int enqueue_frame(const FrameCtx& ctx, const cos_capon_desc& ds, const cos_cf32*
   a.R = (float2*)R;
   a.Rstride = ds.L * ds.L;
   a.refine = ds.refine ? 1 : 0;
-  a.mfma_gram = ds.cov_mfma == 2 ? 1 : 0;
+  a.mfma_gram = ds.cov_mfma == 2 ? 1 : ds.cov_mfma == 3 ? 2 : ds.cov_mfma == 4 ? 3 : 0;
   const char* why = nullptr;
   hipError_t e;
   const int Nzp = ds.Nz - 2 * ds.Yavg;

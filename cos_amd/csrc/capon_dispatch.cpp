@@ -3,7 +3,7 @@
 
 namespace cosk {
 
-template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD, bool REF = false, bool MG = false>
+template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD, bool REF = false, int MG = 0>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s);
 
 template <int G, int CPL>
@@ -18,9 +18,11 @@ static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
   }
   // the optional variants live in the runtime-window kernels
   if (a.fb || a.zmode || a.pmode || a.Rin) return launch_one<float, G, CPL, 0, false, false, 0>(a, Mmax, s);
-  if constexpr (G == 16 && CPL == 2) {   // covariance from per-snapshot Grams on the matrix cores
-    if (TW <= 5 && a.mfma_gram && !a.R && !a.cov_only)
-      return launch_one<float, 16, 2, 5, false, false, 0, false, true>(a, Mmax, s);
+  if constexpr (G == 16 && CPL == 2) {   // covariance work on the matrix cores inside the kernel
+    if (TW <= 5 && a.mfma_gram == 1 && !a.R && !a.cov_only)
+      return launch_one<float, 16, 2, 5, false, false, 0, false, 1>(a, Mmax, s);
+    if (TW <= 5 && (a.mfma_gram == 0 || a.mfma_gram == 2))
+      return launch_one<float, 16, 2, 5, false, false, 0, false, 2>(a, Mmax, s);
   }
   if (TW <= 5) return launch_one<float, G, CPL, 5, false, false, false>(a, Mmax, s);
   if (TW <= 9) return launch_one<float, G, CPL, 9, false, false, false>(a, Mmax, s);

@@ -104,10 +104,12 @@ template <typename T, int G, int CPL, int TM> struct OccHint {
                                               : (G == 16 && CPL == 1 ? 2 : 1);
 };
 
-// MG: a wave whose four pixels are consecutive rows of one beam (L <= 32) builds their
-// covariances from per-snapshot Grams on the matrix cores (mfma_gram.h) instead of the
-// displacement recursion; the launch's other waves keep the recursion.
-template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0, bool REF = false, bool MG = false>
+// MG: for a wave whose four pixels are consecutive rows of one beam (L <= 32), 1 builds
+// their covariances from per-snapshot Grams on the matrix cores (mfma_gram.h) instead of
+// the displacement recursion; 2 keeps the recursion but forms its rank-two corrections on
+// the FP32 matrix cores and leaves only the prefix along the diagonals to the VALU.  The
+// launch's other waves keep the VALU recursion.
+template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0, bool REF = false, int MG = 0>
 __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, const int GREG,
                                             const int gid0, const int npix) {
   using Gp = Grp<G>;
@@ -160,7 +162,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // windows overlap in NG + 2Y rows, which are staged once; group q then uses rows
   // q .. q+2Y.  Otherwise each group stages its own 2Y+1 rows.
   const int NR = NG + TW - 1;
-  bool shared = false, mg = false;
+  bool shared = false, mg = false, mr = false;
   if constexpr (NG > 1) {
     const int p0 = __builtin_amdgcn_readfirstlane(p);
     const int L0 = __builtin_amdgcn_readfirstlane(L), M0 = __builtin_amdgcn_readfirstlane(M);
@@ -169,7 +171,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     const bool mine = active && p == p0 + q && L == L0 && M == M0 && off == o0 && j0 + NG <= Nzp;
     const bool consec = __all(mine);
     shared = consec && (NR * MP + NR * LMAX * (int)(sizeof(cplx<T>) / sizeof(float2)) <= NG * GREG);
-    if constexpr (MG) mg = consec && L <= 32 && mfma_lds(M - L + 1, NR) <= NG * GREG;
+    if constexpr (MG == 1) mg = consec && L <= 32 && mfma_lds(M - L + 1, NR) <= NG * GREG;
+    if constexpr (MG == 2) mr = shared && L <= 32 && TW <= TM && NR * MP + NR * LMAX + 1024 <= NG * GREG;
   }
   float2* xs = shared ? xs_all + q * MP : xs_all + q * GREG;
   const float2* const wrow0 = a.x + ((size_t)(p - q) / Nzp * a.Nz + (j - q)) * a.Mstride + off;   // (shared, mg)
@@ -179,17 +182,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       pexp = mfma_stage(reinterpret_cast<unsigned*>(xs_all), wrow0, a.Mstride, __builtin_amdgcn_readfirstlane(M), NR,
                         mfma_mpf(__builtin_amdgcn_readfirstlane(M - L + 1)));
   } else if (shared) {
-    const float2* src0 = a.x + ((size_t)(p - q) / Nzp * a.Nz + (j - q)) * a.Mstride + off;
-    for (int t = 0; t < NR; ++t) {
-      const float2* src = src0 + (size_t)t * a.Mstride;
-      for (int m = threadIdx.x; m < MP; m += 64) xs_all[t * MP + m] = (m < M) ? src[m] : make_float2(0.f, 0.f);
-    }
+    stage_rows<8, 2>(xs_all, MP, wrow0, a.Mstride, NR, M, threadIdx.x, 64);
   } else {
     // Stage the 2Y+1 snapshots of the window, zero padded to MP elements.
-    for (int t = 0; t < TW; ++t) {
-      const float2* src = a.x + ((size_t)ix * a.Nz + (j + t)) * a.Mstride + off;
-      for (int m = g; m < MP; m += G) xs[t * MP + m] = (m < M) ? src[m] : make_float2(0.f, 0.f);
-    }
+    stage_rows<(G == 16 ? 5 : 8), (G == 16 ? 4 : 2)>(xs, MP, a.x + ((size_t)ix * a.Nz + j) * a.Mstride + off, a.Mstride,
+                                                     TW, M, g, G);
   }
   __syncthreads();
 
@@ -199,7 +196,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 
   cplx<T> A[NS];
 
-  if constexpr (MG) {
+  if constexpr (MG == 1) {
     if (mg) {
       // pixel q's window sum of the row Grams into the transfer buffer, then into the
       // registers of group q
@@ -351,6 +348,76 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     static_for<0, CPL>([&](auto S_) {
     constexpr int S = decltype(S_)::value; A[slot<G, S, 0>()] = s0[S]; });
 
+    bool rec_done = false;
+    if constexpr (MG == 2) {
+      if (mr) {
+        // the window sums D_q of the rank-two corrections on the matrix cores (mfma_gram.h),
+        // pixel q's into rows 1.. of group q's registers through a 32 x 32 buffer, then the
+        // prefix S[r][c] = S[r-1][c-1] + D[r][c] along the diagonals
+        float2* const Tb = xs_all + NR * MP + NR * LMAX;
+        const int K0 = __builtin_amdgcn_readfirstlane(K);
+        auto take = [&](const f32x16& wr, const f32x16& wi, int qq) {
+          __syncthreads();   // the previous window has been read
+          window_store(Tb, wr, wi);
+          __syncthreads();
+          if (q == qq) {
+            static_for<0, CPL>([&](auto S_) {
+              constexpr int S = decltype(S_)::value;
+              static_for<1, G * (S + 1)>([&](auto R_) {
+                constexpr int R = decltype(R_)::value;
+                const float2 v = Tb[R * 32 + cs[S]];
+                A[slot<G, S, R>()] = {(T)v.x, (T)v.y};
+              });
+            });
+          }
+        };
+        const f32x16 zero = {};
+        f32x16 ur, ui, xr, xi, sr, si;
+        if (TW == 5) {
+          // four windows of five rows out of eight in ten passes (as the Gram tree):
+          // U = E3 + E4, X = U + E2 + E1, S0 = X + E0, S1 = X + E5,
+          // Y = U + E5 + E6 (in U), S2 = Y + E2, S3 = Y + E7
+          rank_pass(xs_all, MP, 3, K0, zero, zero, ur, ui);
+          rank_pass(xs_all, MP, 4, K0, ur, ui, ur, ui);
+          rank_pass(xs_all, MP, 2, K0, ur, ui, xr, xi);
+          rank_pass(xs_all, MP, 1, K0, xr, xi, xr, xi);
+          rank_pass(xs_all, MP, 0, K0, xr, xi, sr, si);
+          take(sr, si, 0);
+          rank_pass(xs_all, MP, 5, K0, xr, xi, xr, xi);
+          take(xr, xi, 1);
+          rank_pass(xs_all, MP, 5, K0, ur, ui, ur, ui);
+          rank_pass(xs_all, MP, 6, K0, ur, ui, ur, ui);
+          rank_pass(xs_all, MP, 2, K0, ur, ui, xr, xi);
+          take(xr, xi, 2);
+          rank_pass(xs_all, MP, 7, K0, ur, ui, ur, ui);
+          take(ur, ui, 3);
+        } else {
+          for (int qq = 0; qq < NG; ++qq) {
+            rank_pass(xs_all, MP, qq, K0, zero, zero, sr, si);
+            for (int t = 1; t < TW; ++t) rank_pass(xs_all, MP, qq + t, K0, sr, si, sr, si);
+            take(sr, si, qq);
+          }
+        }
+        static_for<1, LMAX>([&](auto R_) {
+          constexpr int R = decltype(R_)::value;
+          static_for<0, CPL>([&](auto S_) {
+            constexpr int S = decltype(S_)::value;
+            if constexpr (R < G * (S + 1)) {
+              cplx<T> own = {Gp::ror1(A[slot<G, S, R - 1>()].x), Gp::ror1(A[slot<G, S, R - 1>()].y)};
+              if constexpr (S > 0 && R - 1 < G * S) {
+                const cplx<T> lo = {Gp::ror1(A[slot<G, S - 1, R - 1>()].x),
+                                    Gp::ror1(A[slot<G, S - 1, R - 1>()].y)};
+                if (g == 0) own = lo;
+              }
+              A[slot<G, S, R>()].x += own.x;
+              A[slot<G, S, R>()].y += own.y;
+            }
+          });
+        });
+        rec_done = true;
+      }
+    }
+    if (!rec_done) {
     cplx<T> al[TMR][CPL], be[TMR][CPL];
     if constexpr (TM > 0) {
       static_for<0, TM>([&](auto t_) {
@@ -417,6 +484,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         if constexpr (R < G * (S + 1)) A[slot<G, S, R>()] = nv[S];
       });
     });
+    }
   }
 
   // ---------------- trace, loading, padding ----------------
@@ -753,7 +821,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     const float nanv = __builtin_nanf("");
     if (!REF && active && g == 0 && a.P && (!OPTS || a.pmode == 0)) {
       float pv = (float)(T(1) / (norm * accP));
-      if constexpr (MG) pv = ldexpf(pv, -2 * pexp);   // the solve ran on 2^(2 pexp) S
+      if constexpr (MG == 1) pv = ldexpf(pv, -2 * pexp);   // the solve ran on 2^(2 pexp) S
       a.P[po] = valid ? pv : nanv;
     }
 
@@ -1126,7 +1194,7 @@ __device__ __noinline__ void capon_group_call(const CaponLaunch& a, const int MP
 }
 
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false,
-          bool MG = false>
+          int MG = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
                           amdgpu_waves_per_eu(BWD ? 1 : (REF && G == 64) ? 2 : OccHint<T, G, CPL, TM>::value)))
 void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
@@ -1162,7 +1230,7 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
 
 
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false,
-          bool MG = false>
+          int MG = 0>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   constexpr int NG = 64 / G, LMAX = G * CPL;
   const int MP = ((Mmax + LMAX + 4 + 3) / 4) * 4;   // zero padding covers every lag read

@@ -51,8 +51,9 @@ struct CaponLaunch {
   // conj(S_L[r][c]) for the pixel's L, pixel stride Rstride): the runtime-window FP32
   // kernels then solve with it instead of building the covariance
   const float2* Rin;
-  // nonzero: the 16-lane, two-column FP32 kernel builds the covariances of its waves from
-  // per-snapshot Grams on the matrix cores (windows of up to five rows, no R output)
+  // matrix-core covariance work inside the 16-lane, two-column FP32 kernel (windows of up
+  // to five rows): 1 per-snapshot Grams (no R output); 0 and 2 the recursion's rank-two
+  // corrections summed per window where a wave qualifies; 3 the recursion on the VALU only
   int mfma_gram;
 };
 

@@ -52,6 +52,33 @@ __device__ __forceinline__ double rdlane(double v, int l) {
   return *reinterpret_cast<double*>(&p);
 }
 
+// Copies nrows rows of M elements (row stride sstride) from global memory into LDS rows of
+// MP elements (zero past M), lanes l0, l0 + nl, ... of the caller taking the columns.  The
+// loads of RB rows x CB column chunks are all issued before the first store waits on them:
+// a loop that stores each element as it arrives pays one memory latency per element.
+template <int RB, int CB>
+__device__ __forceinline__ void stage_rows(float2* dst, int MP, const float2* src, size_t sstride, int nrows, int M,
+                                           int l0, int nl) {
+  for (int t0 = 0; t0 < nrows; t0 += RB)
+    for (int m0 = l0; m0 < MP; m0 += nl * CB) {
+      float2 v[RB][CB];
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          const int t = t0 + r, m = m0 + nl * c;
+          v[r][c] = (t < nrows && m < M) ? src[(size_t)t * sstride + m] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          const int t = t0 + r, m = m0 + nl * c;
+          if (t < nrows && m < MP) dst[t * MP + m] = v[r][c];
+        }
+    }
+}
+
 // DPP control words (GFX9 encoding).
 constexpr int DPP_ROW_SHR = 0x110;      // + n, n in 1..15
 constexpr int DPP_ROW_ROR = 0x120;      // + n

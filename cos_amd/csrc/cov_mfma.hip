@@ -67,8 +67,7 @@ __global__ __launch_bounds__(64) void cov_mfma_kernel(const CaponLaunch a, const
   const int K = M - L + 1, N = TW * K;
   const int lane = threadIdx.x;
   const float2* src = a.x + ((size_t)ix * a.Nz + j) * a.Mstride + off;
-  for (int t = 0; t < TW; ++t)
-    for (int m = lane; m < MP; m += 64) rows[t * MP + m] = m < M ? src[(size_t)t * a.Mstride + m] : make_float2(0.f, 0.f);
+  stage_rows<8, 2>(rows, MP, src, a.Mstride, TW, M, lane, 64);
 
   // offset of subarray n = t K + k in the staged rows, t MP + k; past the last subarray
   // (up to a whole number of steps) the zero tail of row 0 (offset M, + column < MP)

@@ -3,5 +3,5 @@
 #include "capon_fused_impl.h"
 
 namespace cosk {
-template hipError_t launch_one<float, 16, 2, 5, false, false, 0, false, true>(const CaponLaunch&, int, hipStream_t);
+template hipError_t launch_one<float, 16, 2, 5, false, false, 0, false, 1>(const CaponLaunch&, int, hipStream_t);
 }  // namespace cosk

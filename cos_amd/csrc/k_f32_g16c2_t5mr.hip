@@ -1,0 +1,7 @@
+// Explicit instantiation of the fused Capon kernel, 16 lanes x 2 columns, windows of up to
+// five rows, the recursion's rank-two corrections on the FP32 matrix cores.
+#include "capon_fused_impl.h"
+
+namespace cosk {
+template hipError_t launch_one<float, 16, 2, 5, false, false, 0, false, 2>(const CaponLaunch&, int, hipStream_t);
+}  // namespace cosk

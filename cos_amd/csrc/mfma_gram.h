@@ -141,6 +141,38 @@ __device__ __forceinline__ void gram_pass(const unsigned* F, int t, int MPF, int
   di = im;
 }
 
+// ---- rank-two corrections of the displacement recursion (FP32 matrix cores) ----------
+// The recursion S[r][c] = S[r-1][c-1] + D[r][c] (capon_fused_impl.h) adds, per window row t,
+//   E_t[r][c] = alpha_t[r] conj(alpha_t[c]) - beta_t[r] conj(beta_t[c]),
+//   alpha_t[r] = x_t[K-1+r], beta_t[r] = x_t[r-1]
+// (framework/beamformer/capon/RealTimeCapon/buildR_kernel.cu:116-137 forms the same terms
+// element by element).  For a 32 x 32 block, E_t is four real rank-two products, each one
+// v_mfma_f32_32x32x2_f32 (FP32 operands and accumulation, no split):
+//   Re E = [ar ai] [ar ai]^T - [br bi] [br bi]^T
+//   Im E = [ai -ar] [ar ai]^T + [-bi br] [br bi]^T
+// with lane l supplying row (and column) l % 32 and the inner index l / 32.  A window sum
+// of such terms replaces the recursion's VALU work; the prefix along the diagonals stays
+// on the VALU.  (Three 16 x 16 blocks on v_mfma_f32_16x16x4_f32, six independent chains
+// per row instead of two, measured slower: 1.617 against 1.536 ms per frame, the operand
+// selection for four inner indices costing more VALU than the shorter chains saved.)
+
+// D = C + E_t for the window row t of the staged rows (row stride MP, K subarrays).
+__device__ __forceinline__ void rank_pass(const float2* rows, int MP, int t, int K, const f32x16& cr,
+                                          const f32x16& ci, f32x16& dr, f32x16& di) {
+  const int i = threadIdx.x & 31;
+  const bool h = (threadIdx.x & 32) != 0;
+  const float2* xr = rows + t * MP;
+  const float2 al = xr[K - 1 + i];
+  const float2 bv = xr[i > 0 ? i - 1 : 0];
+  const float2 be = i > 0 ? bv : make_float2(0.f, 0.f);
+  const float a0 = h ? al.y : al.x, b0 = h ? be.y : be.x;
+  const float a1 = h ? -al.x : al.y, b1 = h ? be.x : -be.y;
+  dr = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, a0, cr, 0, 0, 0);
+  di = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, a0, ci, 0, 0, 0);
+  dr = __builtin_amdgcn_mfma_f32_32x32x2f32(b0, -b0, dr, 0, 0, 0);
+  di = __builtin_amdgcn_mfma_f32_32x32x2f32(b1, b0, di, 0, 0, 0);
+}
+
 // The window (real sr, imaginary si, MFMA layout) into the 32 x 32 buffer Tb[i][j].
 __device__ __forceinline__ void window_store(float2* Tb, const f32x16& sr, const f32x16& si) {
   const int c = threadIdx.x & 31, h = (threadIdx.x >> 5) & 1;

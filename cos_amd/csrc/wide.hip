@@ -39,11 +39,7 @@ __global__ __launch_bounds__(WB) void capon_wide_kernel(const CaponLaunch a, con
   float2* A = sm + TW * M;                    // [L][L] (row r, column c), solve only
   float2* b = A + L * L;                      // [L] right-hand side, then v, then y
   float* inv = reinterpret_cast<float*>(b + L);   // [L] 1/U_kk
-  const float2* src = a.x + ((size_t)ix * a.Nz + j) * a.Mstride + a.off;
-  for (int e = tid; e < TW * M; e += WB) {
-    const int t = e / M, m = e - t * M;
-    rows[e] = src[(size_t)t * a.Mstride + m];
-  }
+  stage_rows<4, 1>(rows, M, a.x + ((size_t)ix * a.Nz + j) * a.Mstride + a.off, a.Mstride, TW, M, tid, WB);
   __syncthreads();
   // tr(S) = sum_t sum_m |x_t[m]|^2 #{(k, l): k + l = m, k < K, l < L}
   float trp = 0.f;

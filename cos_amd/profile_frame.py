@@ -2,10 +2,12 @@
 temporal window +/-2, loading 0.01 trace/L; seeded speckle and scatterers) a few times on
 cuda:0, for a profiler to observe the kernels:
 
-    python -m cos_amd.profile_frame [--cov recursion|fused|split] [--frames N] [--refine]
+    python -m cos_amd.profile_frame [--cov default|valu|fused|rank|split] [--frames N] [--refine]
 
---cov selects the covariance: the FP32 displacement recursion inside the fused kernel (the
-default), per-snapshot Grams on the matrix cores inside the fused kernel (fused), or the
+--cov selects the covariance: the library's default (the FP32 displacement recursion inside
+the fused kernel, its rank-two corrections on the matrix cores where they apply), the
+recursion entirely on the VALU (valu), per-snapshot Grams on the matrix cores inside the
+fused kernel (fused), the rank-two corrections explicitly (rank), or the
 separate matrix-core covariance into R followed by the solve (split).
 """
 from __future__ import annotations
@@ -15,7 +17,7 @@ import argparse
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--cov", default="recursion", choices=["recursion", "fused", "split"])
+    ap.add_argument("--cov", default="default", choices=["default", "valu", "fused", "rank", "split"])
     ap.add_argument("--frames", type=int, default=3)
     ap.add_argument("--refine", action="store_true")
     args = ap.parse_args()
@@ -23,7 +25,7 @@ def main():
     from cos_amd import CaponPlan, synth
     x = torch.from_numpy(synth.config2(256, 2048, 64, seed=2)).cuda()
     plan = CaponPlan()
-    mode = {"recursion": False, "fused": "fused", "split": True}[args.cov]
+    mode = {"default": False, "valu": "valu", "fused": "fused", "rank": "rank", "split": True}[args.cov]
     for _ in range(args.frames):
         plan.run(x, 0.01, 32, 2, cov_mfma=mode, refine=args.refine)
     torch.cuda.synchronize()

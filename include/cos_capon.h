@@ -106,7 +106,8 @@ typedef struct {
     int refine;          /* nonzero: one step of iterative refinement on the FP32 paths, the
                             residual a - R_L y taken from the snapshots themselves (a = 1 or
                             the steering vector; no forward-backward averaging) */
-    int cov_mfma;        /* 0: covariance by the FP32 displacement recursion;
+    int cov_mfma;        /* 0: covariance by the FP32 displacement recursion, its rank-two
+                            corrections on the matrix cores where 3 applies (the default);
                             1: covariance on the matrix cores (BF16 hi/lo split, FP32
                             accumulation) into R (or a plan buffer when R is NULL), then the
                             FP32 solve from it; L <= 64, row tables allowed (R then a plan
@@ -115,7 +116,13 @@ typedef struct {
                             (FP16 hi/lo split of power-of-two scaled rows, FP32 accumulation),
                             shared by the windows of a wave's four consecutive pixels, where
                             it applies (16 lanes x 2 columns, 16 < L <= 32, windows of up to
-                            five rows, default options, no R output); the recursion elsewhere */
+                            five rows, default options, no R output); the recursion elsewhere;
+                            3: the displacement recursion with its rank-two corrections
+                            (two per window row) summed over each window on the FP32 matrix
+                            cores, shared by a wave's four consecutive pixels, where it
+                            applies (16 lanes x 2 columns, 16 < L <= 32, windows of up to
+                            five rows); no split, FP32 products and accumulation;
+                            4: the displacement recursion entirely on the VALU */
 } cos_capon_desc;
 
 enum {

@@ -144,10 +144,13 @@ def _plan_run(x, d, L, Y, **kw):
     return {k: v.cpu().numpy() for k, v in o.items()}, plan
 
 
-def test_cfg2_slice_power_within_1e4_of_fp64(cfg2_slice):
-    say("cfg2 slice fp32", 8, 2048, 64, 32)
+@pytest.mark.parametrize("cov", [False, "valu"])
+def test_cfg2_slice_power_within_1e4_of_fp64(cfg2_slice, cov):
+    """The default covariance (rank-two corrections on the matrix cores) and the recursion
+    entirely on the VALU."""
+    say("cfg2 slice fp32", cov, 8, 2048, 64, 32)
     x, ref = cfg2_slice
-    o, _ = _plan_run(x, 0.01, 32, 2)
+    o, _ = _plan_run(x, 0.01, 32, 2, cov_mfma=cov)
     e = rel(o["P"], ref["P"].ravel())
     print(f"P rel err max {e.max():.3e} p99.9 {np.quantile(e, 0.999):.3e}", flush=True)
     assert e.max() <= 1e-4
@@ -188,16 +191,19 @@ def test_cfg2_slice_refined_power(cfg2_slice):
     assert ze.max() <= 1e-4
 
 
+@pytest.mark.parametrize("mode", ["fused", "rank", "valu"])
 @pytest.mark.parametrize("M,L,Y,Nx,Nz", [(64, 32, 2, 2, 40), (48, 24, 1, 3, 37), (40, 20, 0, 2, 30), (80, 32, 2, 2, 44),
                                        (33, 17, 2, 1, 21), (64, 32, 2, 3, 43)])
-def test_fused_mfma_covariance_matches_oracle(M, L, Y, Nx, Nz):
-    """The covariance from per-snapshot Grams on the matrix cores inside the fused kernel
-    (FP16 hi/lo split of power-of-two scaled rows): P, w and z against the FP64 oracle, on
-    shapes whose waves take the matrix-core path (four consecutive rows) and, where the
-    row count is not a multiple of four, the recursion at the beam ends."""
-    say("fused mfma covariance", M, L, Y, Nx, Nz)
+def test_fused_mfma_covariance_matches_oracle(M, L, Y, Nx, Nz, mode):
+    """Covariance work on the matrix cores inside the fused kernel ("fused": per-snapshot
+    Grams from an FP16 hi/lo split of power-of-two scaled rows; "rank": the recursion's
+    rank-two corrections summed per window in FP32; "valu": none, the recursion on the
+    VALU): P, w and z against the FP64 oracle, on shapes whose waves take the matrix-core
+    path (four consecutive rows) and, where the row count is not a multiple of four, the
+    recursion at the beam ends."""
+    say("fused mfma covariance", mode, M, L, Y, Nx, Nz)
     x = uniform_cube(M * 7 + L + Y, Nx, Nz, M)
-    o, _ = _plan_run(x, 0.05, L, Y, cov_mfma="fused")
+    o, _ = _plan_run(x, 0.05, L, Y, cov_mfma=mode)
     ref = O.capon(x.astype(np.complex128), 0.05, L, Y)
     eP = rel(o["P"], ref["P"])
     we = np.linalg.norm(o["w"] - ref["w"], axis=1) / np.linalg.norm(ref["w"], axis=1)
@@ -206,15 +212,32 @@ def test_fused_mfma_covariance_matches_oracle(M, L, Y, Nx, Nz):
     assert eP.max() <= 1e-5 and we.max() <= 1e-5 and ez <= 1e-5
 
 
-def test_fused_mfma_covariance_headline_slice(cfg2_slice):
+@pytest.mark.parametrize("mode", ["fused", "rank"])
+def test_fused_mfma_covariance_headline_slice(cfg2_slice, mode):
     """The 64-element, L = 32 slice (speckle and 30 dB scatterers spanning 2^5 in amplitude
     within a wave's rows) through the matrix-core covariance: power within 1e-5 of FP64."""
-    say("fused mfma covariance cfg2 slice", 8, 2048, 64, 32)
+    say("fused mfma covariance cfg2 slice", mode, 8, 2048, 64, 32)
     x, ref = cfg2_slice
-    o, _ = _plan_run(x, 0.01, 32, 2, cov_mfma="fused")
+    o, _ = _plan_run(x, 0.01, 32, 2, cov_mfma=mode)
     e = rel(o["P"], ref["P"].ravel())
     print(f"P rel err max {e.max():.3e}", flush=True)
     assert e.max() <= 1e-5
+
+
+def test_rank_corrections_covariance_output():
+    """The rank-two corrections on the matrix cores with the materialised R (the
+    reference layout) against the oracle's."""
+    import torch
+    from cos_amd import CaponPlan
+    say("rank covariance R + refine", 64, 32, 2)
+    x = uniform_cube(91, 2, 30, 64)
+    xt = torch.from_numpy(x).cuda()
+    o = CaponPlan().run(xt, 0.05, 32, 2, cov_mfma="rank", want=("P", "z", "w", "R"))
+    torch.cuda.synchronize()
+    ref = O.reference_gpu_outputs(x.astype(np.complex128), float(np.float32(0.05)), 32, 2)
+    R = o["R"].cpu().numpy().reshape(ref["R"].shape)
+    assert np.abs(R - ref["R"]).max() <= 2e-6 * np.abs(ref["R"]).max() * 8
+    assert rel(o["P"].cpu().numpy(), ref["P"].ravel()).max() <= 1e-5
 
 
 def test_refinement_on_ill_conditioned_covariance():
