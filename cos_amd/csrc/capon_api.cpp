@@ -134,6 +134,7 @@ struct cos_capon_plan {
   FrameGraph fg;
   DevBuf rows;                // table-driven frames: per-bucket counts [8], then row lists [nb][Nzp]
   bool fallback_ran = false;
+  int fb_lists = 1;     // FP64 fallback lists of the last frame (one per lane shape with row tables)
   unsigned buf_gen = 0;       // bumped by every reallocation of a plan buffer
   cos_capon_plan() {
     for (DevBuf* b : {&flags, &list, &count, &x, &z, &w, &R, &P, &B, &bs_tw, &rows}) b->gen = &buf_gen;
@@ -287,6 +288,7 @@ struct FrameCtx {
 };
 
 // Enqueue one frame on stream s (arguments already validated; scratch already sized).
+int fb_lists(const cos_capon_desc& ds);
 int enqueue_frame(const FrameCtx& c, const cos_capon_desc& ds, const cos_cf32* x, float* P, cos_cf32* z,
                   cos_cf32* w, int wstride, cos_cf32* R, unsigned char* flags, hipStream_t s, bool fb_needed);
 
@@ -410,32 +412,52 @@ int enqueue_frame(const FrameCtx& ctx, const cos_capon_desc& ds, const cos_cf32*
     if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP32 kernel");
   }
   if (!fb) return 0;
-  e = cosk::compact_flags(fl, a.npix, ctx.list, ctx.count, s);
+  // With row tables the flagged pixels go to one list per FP64 lane shape (L <= 16, 32, 64)
+  // by their row's L, so short rows run four to a wave instead of one.
+  const int nfb = fb_lists(ds);
+  cosk::RowBuckets fbk{};
+  for (int L = 0; L <= 64; ++L) fbk.b[L] = (unsigned char)std::min(nfb - 1, L <= 16 ? 0 : L <= 32 ? 1 : 2);
+  const int fcap[3] = {16, 32, 64};
+  e = nfb > 1 ? cosk::compact_flags_rows(fl, a.npix, ctx.list, ctx.count, ds.row_L, ds.Nz - 2 * ds.Yavg, ds.Yavg,
+                                         fbk, nfb, a.npix, s)
+              : cosk::compact_flags(fl, a.npix, ctx.list, ctx.count, s);
   if (e != hipSuccess) return fail_hip(e, "compaction");
-  // The FP64 pass over the device-side list: the entries the host expects (the bound's
+  // The FP64 pass over each device-side list: the entries the host expects (the bound's
   // count plus a slack for FP32 breakdowns) one block per group, the out-of-line
   // persistent kernel only over entries past them (none in the usual frame).  The split
   // changes only the launch shapes: every list entry is covered whatever the count.
   constexpr long kSlack = 4096;
   const int head = (int)std::min<long>(a.npix, ctx.fb_bound + kSlack);
-  cosk::CaponLaunch f = a;
-  f.flags = nullptr;
-  f.plist = ctx.list;
-  f.pcount = ctx.count;
-  f.pbase = 0;
-  f.npix = head;
-  e = cosk::launch_capon_f64_list(f, Lmax, Mmax, false, s, &why);
-  if (e == hipSuccess && head < a.npix) {
-    f.plist += head;
-    f.pbase = head;
-    f.npix = a.npix - head;
-    e = cosk::launch_capon_f64_list(f, Lmax, Mmax, true, s, &why);
+  for (int b = 0; b < nfb; ++b) {
+    const int cap = nfb > 1 ? fcap[b] : Lmax;
+    cosk::CaponLaunch f = a;
+    f.flags = nullptr;
+    f.plist = ctx.list + (size_t)b * a.npix;
+    f.pcount = ctx.count + b;
+    f.pbase = 0;
+    f.npix = head;
+    f.Lcap = cap;
+    e = cosk::launch_capon_f64_list(f, cap, Mmax, false, s, &why);
+    if (e == hipSuccess && head < a.npix) {
+      f.plist += head;
+      f.pbase = head;
+      f.npix = a.npix - head;
+      e = cosk::launch_capon_f64_list(f, cap, Mmax, true, s, &why);
+    }
+    if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 fallback");
   }
-  if (e != hipSuccess) return why ? fail(e, why) : fail_hip(e, "FP64 fallback");
   return 0;
 }
 
 bool same_desc(const cos_capon_desc& a, const cos_capon_desc& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
+// FP64 fallback lists of a frame: one per lane shape up to the frame's largest L with row
+// tables, one otherwise.
+int fb_lists(const cos_capon_desc& ds) {
+  const bool tables = ds.row_L || ds.row_M || ds.row_off || ds.row_d;
+  if (!tables) return 1;
+  return ds.Lmax <= 16 ? 1 : ds.Lmax <= 32 ? 2 : 3;
+}
 
 }  // namespace
 
@@ -460,8 +482,10 @@ int cos_capon_run(cos_capon_plan* plan, const cos_capon_desc* desc, const cos_cf
   }
   if (fb) {
     if (!flags && (e = plan->flags.ensure(npix)) != hipSuccess) return fail_hip(e, "flags buffer");
-    if ((e = plan->list.ensure(sizeof(int) * (size_t)npix)) != hipSuccess) return fail_hip(e, "list buffer");
-    if ((e = plan->count.ensure(sizeof(int))) != hipSuccess) return fail_hip(e, "count buffer");
+    const int nfb = fb_lists(ds);
+    if ((e = plan->list.ensure(sizeof(int) * (size_t)npix * nfb)) != hipSuccess) return fail_hip(e, "list buffer");
+    if ((e = plan->count.ensure(sizeof(int) * nfb)) != hipSuccess) return fail_hip(e, "count buffer");
+    plan->fb_lists = nfb;
   }
   plan->last_stream = s;
   plan->fallback_ran = fb;
@@ -928,8 +952,10 @@ int cos_capon_last_fallback_count(cos_capon_plan* plan, int* count) {
   if (!plan->fallback_ran) return 0;
   hipError_t e = hipStreamSynchronize(plan->last_stream);
   if (e != hipSuccess) return fail_hip(e, "synchronize");
-  e = hipMemcpy(count, plan->count.p, sizeof(int), hipMemcpyDeviceToHost);
+  int c[3] = {0, 0, 0};
+  e = hipMemcpy(c, plan->count.p, sizeof(int) * plan->fb_lists, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return fail_hip(e, "copy count");
+  *count = c[0] + c[1] + c[2];
   return 0;
 }
 

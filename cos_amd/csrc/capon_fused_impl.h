@@ -621,6 +621,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // 64-lane groups from registers; the FP64 16-lane groups publish row k in LDS for the
   // rows past the first CRR
   constexpr bool DPP_FMA = std::is_same<T, float>::value && G == 16;
+  // one pixel per wave in FP32 with 64 rows: the trailing block of rows and columns 32..63
+  // is updated by the first 32 steps at once on the matrix cores
+  constexpr bool BLK = std::is_same<T, float>::value && G == 64 && CPL == 1;
   constexpr int CRR = CholRegRows<G>::value;
   constexpr bool PUBLISH = !DPP_FMA && CRR < LMAX - 1;
   cplx<T>* const Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
@@ -702,18 +705,56 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           if constexpr (i < G * (S + 1)) sub_conj_mul(A[slot<G, S, i>()], u, A[slot<G, S, k>()]);
         });
       };
+      // BLK: the steps of the first 32 rows update rows below 32 only; the block of rows
+      // and columns 32.. takes their 32 rank-one terms at once on the matrix cores below
+      constexpr int IEND = (BLK && k < 32) ? 32 : LMAX;
       if constexpr (SKIP) {
         // the wave steps only to its own L: rows in blocks of four with one uniform test per
         // block (a test per row cost a scalar compare and branch for every step and row);
         // rows at or past L in the last block meet U[k][i] = 0 from the identity padding
         // and change nothing
-        static_for<0, (LMAX - k - 1 + 3) / 4>([&](auto B_) {
+        static_for<0, (IEND - k - 1 + 3) / 4>([&](auto B_) {
           constexpr int i0 = k + 1 + 4 * decltype(B_)::value;
           if (i0 >= Lw) return;
-          static_for<i0, (i0 + 4 < LMAX ? i0 + 4 : LMAX)>(row_update);
+          static_for<i0, (i0 + 4 < IEND ? i0 + 4 : IEND)>(row_update);
         });
       } else {
-        static_for<k + 1, LMAX>(row_update);
+        static_for<k + 1, IEND>(row_update);
+      }
+    }
+    if constexpr (BLK && k == 31) {
+      if (Lw > 32) {
+        // S22 -= U12^H U12 for rows and columns 32..63, U12 = rows 0..31 of U in the lanes
+        // of columns 32..63: per row m, Re += [Ur Ui] [Ur Ui]^T, Im += [-Ui Ur] [Ur Ui]^T on
+        // v_mfma_f32_32x32x2_f32, lane l < 32 supplying column 32 + l's real part (moved
+        // down by a half-wave swap) and lane 32 + l its own imaginary part.  Lanes of
+        // columns below 32 receive values in rows 32.. of their columns, below the
+        // diagonal, which nothing reads.
+        const bool lo = (threadIdx.x & 32) == 0;
+        f32x16 cr = {}, ci = {};
+        static_for<0, 32>([&](auto M_) {
+          constexpr int m = decltype(M_)::value;
+          const unsigned ur = __float_as_uint((float)A[m].x), ui = __float_as_uint((float)A[m].y);
+          const float urd = __uint_as_float(__builtin_amdgcn_permlane32_swap(ur, ur, false, false)[1]);
+          const float uid = __uint_as_float(__builtin_amdgcn_permlane32_swap(ui, ui, false, false)[1]);
+          const float X = lo ? urd : (float)A[m].y;
+          const float Y = lo ? -uid : (float)A[m].x;
+          cr = __builtin_amdgcn_mfma_f32_32x32x2f32(X, X, cr, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_32x32x2f32(Y, X, ci, 0, 0, 0);
+        });
+        // accumulator element v of lane l: column l % 32, row 8(v/4) + 4(l/32) + v%4 of the
+        // block; the lower half's rows move up to the lanes of their columns
+        static_for<0, 16>([&](auto V_) {
+          constexpr int v = decltype(V_)::value;
+          constexpr int rhi = 32 + 8 * (v / 4) + 4 + v % 4, rlo = 32 + 8 * (v / 4) + v % 4;
+          const unsigned br = __float_as_uint(cr[v]), bi = __float_as_uint(ci[v]);
+          const float lr = __uint_as_float(__builtin_amdgcn_permlane32_swap(br, br, false, false)[0]);
+          const float li = __uint_as_float(__builtin_amdgcn_permlane32_swap(bi, bi, false, false)[0]);
+          A[rhi].x -= (T)cr[v];
+          A[rhi].y -= (T)ci[v];
+          A[rlo].x -= (T)lr;
+          A[rlo].y -= (T)li;
+        });
       }
     }
   });

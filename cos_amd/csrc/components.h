@@ -3,10 +3,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "capon_kernels.h"
+
 namespace cosk {
 
 hipError_t fill_i32(int* p, int n, int v, hipStream_t s);
 hipError_t compact_flags(const uint8_t* flags, int n, int* list, int* count, hipStream_t s);
+// the same into nb lists by the bucket bk.b[L] of each pixel's row L (rowL[j + Y] for
+// output row j = pixel % nzp): list b at list + b * lstride, its length in count[b]
+hipError_t compact_flags_rows(const uint8_t* flags, int n, int* list, int* count, const int* rowL, int nzp, int Y,
+                              RowBuckets bk, int nb, int lstride, hipStream_t s);
 hipError_t solve_lu(const float2* A, const float2* b, float2* x, int N, int batch, hipStream_t s);
 hipError_t buildr_beamspace(const float2* xbs, float2* R, float d, int Nb, int K, int Y, int Nx,
                             int Nz, hipStream_t s);

@@ -13,10 +13,14 @@ __global__ void fill_i32_kernel(int* p, int n, int v) {
 }
 
 // Pixels with any flag bit go to the FP64 list (order is irrelevant: pixels are
-// independent and each writes only its own outputs).  Each lane tests four pixels (one
-// 32-bit load where the four are in range), and each wave reserves its entries with one
-// atomic, ranked by ballot.
-__global__ __launch_bounds__(256) void compact_flags_kernel(const uint8_t* flags, int n, int* list, int* count) {
+// independent and each writes only its own outputs); with a row-L table, to the list of
+// the bucket of their row's L (list b at list + b * lstride, its length in count[b]), so
+// each bucket runs on the lane shape of its L.  Each lane tests four pixels (one 32-bit
+// load where the four are in range), and each wave reserves its entries with one atomic
+// per bucket, ranked by ballot.
+__global__ __launch_bounds__(256) void compact_flags_kernel(const uint8_t* flags, int n, int* list, int* count,
+                                                            const int* rowL, int nzp, int Y, RowBuckets bk, int nb,
+                                                            int lstride) {
   const int i0 = 4 * (blockIdx.x * 256 + threadIdx.x);
   uint32_t f4 = 0;
   if (i0 + 3 < n && ((uintptr_t)flags & 3) == 0) {
@@ -28,12 +32,21 @@ __global__ __launch_bounds__(256) void compact_flags_kernel(const uint8_t* flags
   const int lane = threadIdx.x & 63;
   for (int u = 0; u < 4; ++u) {
     const bool hit = ((f4 >> (8 * u)) & 0xffu) != 0;
-    const unsigned long long m = __ballot(hit);
-    if (m == 0) continue;
-    int base = 0;
-    if (lane == __ffsll((long long)m) - 1) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, __ffsll((long long)m) - 1);
-    if (hit) list[base + __popcll(m & ((1ull << lane) - 1ull))] = i0 + u;
+    int b = 0;
+    if (rowL && hit) {
+      int L = rowL[(i0 + u) % nzp + Y];
+      L = L < 0 ? 0 : L > 64 ? 64 : L;
+      b = bk.b[L];
+    }
+    for (int bb = 0; bb < nb; ++bb) {
+      const bool mine = hit && b == bb;
+      const unsigned long long m = __ballot(mine);
+      if (m == 0) continue;
+      int base = 0;
+      if (lane == __ffsll((long long)m) - 1) base = atomicAdd(count + bb, __popcll(m));
+      base = __shfl(base, __ffsll((long long)m) - 1);
+      if (mine) list[(size_t)bb * lstride + base + __popcll(m & ((1ull << lane) - 1ull))] = i0 + u;
+    }
   }
 }
 
@@ -95,9 +108,19 @@ hipError_t fill_i32(int* p, int n, int v, hipStream_t s) {
 }
 
 hipError_t compact_flags(const uint8_t* flags, int n, int* list, int* count, hipStream_t s) {
-  hipError_t e = fill_i32(count, 1, 0, s);
+  return compact_flags_rows(flags, n, list, count, nullptr, 1, 0, RowBuckets{}, 1, 0, s);
+}
+
+hipError_t compact_flags_rows(const uint8_t* flags, int n, int* list, int* count, const int* rowL, int nzp, int Y,
+                              RowBuckets bk, int nb, int lstride, hipStream_t s) {
+  if (nb < 1 || nb > 8 || (rowL && nzp < 1)) return hipErrorInvalidValue;
+  if (rowL)
+    for (int L = 0; L <= 64; ++L)
+      if (bk.b[L] >= nb) return hipErrorInvalidValue;
+  hipError_t e = fill_i32(count, nb, 0, s);
   if (e != hipSuccess || n <= 0) return e;
-  hipLaunchKernelGGL(compact_flags_kernel, dim3((n + 1023) / 1024), dim3(256), 0, s, flags, n, list, count);
+  hipLaunchKernelGGL(compact_flags_kernel, dim3((n + 1023) / 1024), dim3(256), 0, s, flags, n, list, count, rowL,
+                     nzp, Y, bk, nb, lstride);
   return hipGetLastError();
 }
 
