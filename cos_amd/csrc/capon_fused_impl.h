@@ -334,13 +334,29 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         });
       }
     } else {
+      // eight subarrays at a time with every LDS read of the batch issued before the
+      // products (read one at a time, each waited on, the loop paid the LDS latency per
+      // subarray); past K the shared value is zero and the column read stays in the row
       for (int t = 0; t < TW; ++t) {
         const float2* xr = xs + t * MP;
-        for (int k = 0; k < K; ++k) {
-          const cplx<T> u = ldx<T>(xr + k);
-          static_for<0, CPL>([&](auto S_) {
-            constexpr int S = decltype(S_)::value;
-            add_mul_conj(s0[S], u, ldx<T>(xr + k + cs[S]));
+        for (int k0 = 0; k0 < K; k0 += 8) {
+          cplx<T> u[8], v[8][CPL];
+          static_for<0, 8>([&](auto J_) {
+            constexpr int jj = decltype(J_)::value;
+            const int kk = min(k0 + jj, K - 1);
+            u[jj] = ldx<T>(xr + kk);
+            if (k0 + jj >= K) u[jj] = {T(0), T(0)};
+            static_for<0, CPL>([&](auto S_) {
+              constexpr int S = decltype(S_)::value;
+              v[jj][S] = ldx<T>(xr + kk + cs[S]);
+            });
+          });
+          static_for<0, 8>([&](auto J_) {
+            constexpr int jj = decltype(J_)::value;
+            static_for<0, CPL>([&](auto S_) {
+              constexpr int S = decltype(S_)::value;
+              add_mul_conj(s0[S], u[jj], v[jj][S]);
+            });
           });
         }
       }
@@ -451,8 +467,17 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         }
       });
       auto corr = [&](int t, const cplx<T>* alt, const cplx<T>* bet) {
-        const cplx<T> u1 = ldx<T>(xs + t * MP + K - 1 + R);
-        const cplx<T> u0 = ldx<T>(xs + t * MP + R - 1);
+        // one pixel per wave with the window in registers: u1 = x_t[K-1+R] and
+        // u0 = x_t[R-1] are alpha_t and beta_t of lane R (read into scalars, no LDS
+        // round trip; every lane's read of the row would be the same address)
+        cplx<T> u1, u0;
+        if constexpr (G == 64 && TM > 0) {
+          u1 = bcastc<G, R>(alt[0]);
+          u0 = bcastc<G, R>(bet[0]);
+        } else {
+          u1 = ldx<T>(xs + t * MP + K - 1 + R);
+          u0 = ldx<T>(xs + t * MP + R - 1);
+        }
         static_for<0, CPL>([&](auto S_) {
     constexpr int S = decltype(S_)::value;
           if constexpr (R < G * (S + 1)) {
@@ -639,9 +664,45 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     constexpr int S = decltype(S_)::value;
     b1[S] = {cs[S] < L ? T(1) : T(0), T(0)};
   });
+  // FP32 16-lane groups with three columns per lane: the steps of row block kb (rows
+  // 16kb .. 16kb+15) update only rows of that block on the VALU; the 16 x 16 blocks (ib, s)
+  // of rows 16ib.. of slot s, ib > kb, take their rank-one terms on the matrix cores
+  // (blk_op), in the accumulator layout from step 0 until block ib's pivots begin.  An FP32
+  // MFMA occupies the SIMD for its 32 cycles (no VALU issues beside it, measured), and
+  // 16 x 16 x 1 does the work of 16 FMA instructions: a win for this shape, which runs one
+  // wave per SIMD and so issues at most one VALU instruction per four cycles, and neutral
+  // for two columns per lane (two waves per SIMD interleave their VALU work at about 2.6
+  // cycles per instruction; 1.40 ms per headline frame either way), which keeps the VALU.
+  constexpr bool MBLK = DPP_FMA && CPL >= 3;
+  constexpr int NBK = MBLK ? CPL * (CPL - 1) / 2 : 1;
+  f32x16 bkr[NBK], bki[NBK];
+  auto for_blocks = [&](auto&& f) {   // f(ic<b>, ic<ib>, ic<s>) for every block
+    if constexpr (MBLK) {
+      static_for<1, CPL>([&](auto IB_) {
+        constexpr int ib = decltype(IB_)::value;
+        static_for<ib, CPL>([&](auto S_) {
+          constexpr int s = decltype(S_)::value;
+          constexpr int b = (ib - 1) * CPL - (ib - 1) * ib / 2 + (s - ib);
+          f(ic<b>{}, ic<ib>{}, ic<s>{});
+        });
+      });
+    }
+  };
   static_for<0, LMAX>([&](auto Kk_) {
     constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
     if (SKIP && k >= Lw) return;
+    if constexpr (MBLK && k == 0) {
+      for_blocks([&](auto B_, auto IB_, auto S_) {
+        constexpr int b = decltype(B_)::value, ib = decltype(IB_)::value, s = decltype(S_)::value;
+        static_for<0, 16>([&](auto N_) {
+          constexpr int n = decltype(N_)::value;
+          bkr[b][n] = A[slot<G, s, 16 * ib + n>()].x;
+          bki[b][n] = A[slot<G, s, 16 * ib + n>()].y;
+        });
+        xpose_groups(bkr[b]);
+        xpose_groups(bki[b]);
+      });
+    }
     // A pivot that is not positive (or NaN) is clamped to tiny; the flag pass detects
     // the clamp from the largest 1/U_kk.  The loading is added in the identity padding
     // too, where it changes nothing (its right-hand side is 0).
@@ -674,11 +735,32 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       // reads each slot needs the DPP read's wait states, the later ones follow at least one
       // block of FMAs that does not write it.
       // Row i lives in slots si .. CPL-1; they are updated two at a time sharing the
-      // broadcast, an odd one alone.
-      static_for<k + 1, LMAX>([&](auto Ii_) {
+      // broadcast, an odd one alone.  With MBLK the rows past this step's row block are
+      // updated on the matrix cores: four MFMAs per block, two dependent pairs.  A wave
+      // issues in order, so an MFMA issued while the matrix pipe still runs its previous one
+      // holds back the VALU work behind it; one is issued before each row's FMAs (at least
+      // eight VALU instructions, the 32 cycles an MFMA takes), pinned there by scheduling
+      // barriers, and the ones left over after the step's last row.
+      constexpr int IEND = MBLK ? G * (k / G + 1) : LMAX;
+      auto mfma_op = [&](auto M_) {
+        constexpr int m = decltype(M_)::value;
+        for_blocks([&](auto B_, auto IB_, auto S_) {
+          constexpr int b = decltype(B_)::value, ib = decltype(IB_)::value, s = decltype(S_)::value;
+          // blocks are numbered by ib, so the ones still live at step k come last
+          if constexpr (k < 16 * ib && m / 4 == b - (NBK - nlive<CPL>(k))) {
+            __builtin_amdgcn_sched_barrier(0);
+            blk_op<m % 4>(bkr[b], bki[b], A[slot<G, ib, k>()].x, A[slot<G, ib, k>()].y, A[slot<G, s, k>()].x,
+                          A[slot<G, s, k>()].y);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        });
+      };
+      constexpr int NM = MBLK ? 4 * nlive<CPL>(k) : 0;
+      static_for<k + 1, IEND>([&](auto Ii_) {
         constexpr int i = decltype(Ii_)::value, si = i / G, li = i % G;
         constexpr bool first = i == k + 1 || i % G == 0;
         constexpr int ns = CPL - si;
+        if constexpr (i - k - 1 < NM) mfma_op(ic<i - k - 1>{});
         if constexpr (ns % 2 == 1)
           sub_conj_bcast_mul<li, first>(A[slot<G, si, i>()], A[slot<G, si, k>()], A[slot<G, si, k>()]);
         static_for<0, ns / 2>([&](auto P_) {
@@ -688,6 +770,21 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
                                                                     A[slot<G, s0 + 1, k>()]);
         });
       });
+      static_for<(IEND - k - 1 < NM ? IEND - k - 1 : NM), NM>([&](auto M_) { mfma_op(M_); });
+      if constexpr (MBLK) {
+        // a block whose row block's pivots come next returns to the slot layout
+        for_blocks([&](auto B_, auto IB_, auto S_) {
+          constexpr int b = decltype(B_)::value, ib = decltype(IB_)::value, s = decltype(S_)::value;
+          if constexpr (k == 16 * ib - 1) {
+            xpose_groups(bkr[b]);
+            xpose_groups(bki[b]);
+            static_for<0, 16>([&](auto N_) {
+              constexpr int n = decltype(N_)::value;
+              A[slot<G, s, 16 * ib + n>()] = {bkr[b][n], bki[b][n]};
+            });
+          }
+        });
+      }
     } else {
       // the first rows take U[k][i] by a register broadcast, which covers the latency of
       // the LDS reads for the rest

@@ -173,6 +173,58 @@ __device__ __forceinline__ void rank_pass(const float2* rows, int MP, int t, int
   di = __builtin_amdgcn_mfma_f32_32x32x2f32(b1, b0, di, 0, 0, 0);
 }
 
+// ---- 16 x 16 blocks of the Cholesky trailing update (FP32 matrix cores) ----------------
+// In the fused kernel's 16-lane groups (four pixels per wave), lane j of group q holds
+// column j of a 16-column slot and row n of a 16-row block in register n: X_q[n][j] at
+// (group q, register n).  v_mfma_f32_16x16x1f32 computes four independent 16 x 16
+// rank-one products, one per group: block b takes A[i] from lane 16b + i and B[j] from lane
+// 16b + j (so row k of the slot, in every group, is both operands as it stands) and keeps
+// D_b[i][j] in group i/4, lane j, accumulator element (i%4) + 4b.  With n = 4h + r the two
+// layouts differ by swapping the group index with h for each r: a 4 x 4 transpose of
+// 16-lane chunks, done by two 32-lane swaps (bit 1 of group and h) and two 16-lane swaps
+// (bit 0), and its own inverse.  Register n of the slot layout is element n of the
+// accumulator layout.
+
+__device__ __forceinline__ void xpose_groups(f32x16& v) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {   // groups 2,3 of chunk h <-> groups 0,1 of chunk h + 2
+      const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[4 * h + r]), __float_as_uint(v[4 * h + 8 + r]),
+                                                      false, false);
+      v[4 * h + r] = __uint_as_float(s[0]);
+      v[4 * h + 8 + r] = __uint_as_float(s[1]);
+    }
+#pragma unroll
+    for (int h = 0; h < 4; h += 2) {   // odd groups of chunk h <-> even groups of chunk h + 1
+      const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[4 * h + r]), __float_as_uint(v[4 * h + 4 + r]),
+                                                      false, false);
+      v[4 * h + r] = __uint_as_float(s[0]);
+      v[4 * h + 4 + r] = __uint_as_float(s[1]);
+    }
+  }
+}
+
+// D[i][j] -= conj(u_i) v_j for the four groups' 16 x 16 blocks (accumulator layout):
+// Re -= ur_i vr_j + ui_i vi_j, Im -= ur_i vi_j - ui_i vr_j, u from the block's rows, v from
+// its columns (row k of the respective slots); one of the four MFMAs per call, J = 0..3.
+template <int J>
+__device__ __forceinline__ void blk_op(f32x16& dr, f32x16& di, float ur, float ui, float vr, float vi) {
+  if constexpr (J == 0) dr = __builtin_amdgcn_mfma_f32_16x16x1f32(-ur, vr, dr, 0, 0, 0);
+  if constexpr (J == 1) di = __builtin_amdgcn_mfma_f32_16x16x1f32(-ur, vi, di, 0, 0, 0);
+  if constexpr (J == 2) dr = __builtin_amdgcn_mfma_f32_16x16x1f32(-ui, vi, dr, 0, 0, 0);
+  if constexpr (J == 3) di = __builtin_amdgcn_mfma_f32_16x16x1f32(ui, vr, di, 0, 0, 0);
+}
+
+// Number of 16 x 16 blocks (ib, s), 1 <= ib <= s < CPL, still updated on the matrix cores at
+// Cholesky step k (those with 16 ib > k).
+template <int CPL>
+__host__ __device__ constexpr int nlive(int k) {
+  int n = 0;
+  for (int ib = k / 16 + 1; ib < CPL; ++ib) n += CPL - ib;
+  return n;
+}
+
 // The window (real sr, imaginary si, MFMA layout) into the 32 x 32 buffer Tb[i][j].
 __device__ __forceinline__ void window_store(float2* Tb, const f32x16& sr, const f32x16& si) {
   const int c = threadIdx.x & 31, h = (threadIdx.x >> 5) & 1;
