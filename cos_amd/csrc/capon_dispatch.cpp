@@ -48,10 +48,19 @@ hipError_t launch_capon_f32(const CaponLaunch& a, int Lmax, int Mmax, hipStream_
 }
 
 static hipError_t f64_launch(const CaponLaunch& a, int Lmax, int Mmax, bool list, hipStream_t s, const char** why) {
-  if (Lmax <= 16)
+  // L <= 32: the plain kernel (direct covariance sums, 165 complex MACs per element at
+  // M = 64, L = 32, Y = 2) for whole frames, the same-GPU FP64 comparator; over the fallback
+  // pixel list the covariance by the FP64 displacement recursion (10 per element)
+  if (Lmax <= 16) {
+    if (a.plist)
+      return list ? launch_one<double, 16, 1, 0, false, true, false>(a, Mmax, s) : launch_one<double, 16, 1, 0, false, false, false>(a, Mmax, s);
     return list ? launch_one<double, 16, 1, 0, true, true, false>(a, Mmax, s) : launch_one<double, 16, 1, 0, true, false, false>(a, Mmax, s);
-  if (Lmax <= 32)
+  }
+  if (Lmax <= 32) {
+    if (a.plist)
+      return list ? launch_one<double, 16, 2, 0, false, true, false>(a, Mmax, s) : launch_one<double, 16, 2, 0, false, false, false>(a, Mmax, s);
     return list ? launch_one<double, 16, 2, 0, true, true, false>(a, Mmax, s) : launch_one<double, 16, 2, 0, true, false, false>(a, Mmax, s);
+  }
   if (Lmax <= 64)   // covariance by the FP64 recursion, the columns in registers
     return list ? launch_one<double, 64, 1, 0, false, true, false>(a, Mmax, s) : launch_one<double, 64, 1, 0, false, false, false>(a, Mmax, s);
   if (why) *why = "L > 64 is not supported by the FP64 kernel";

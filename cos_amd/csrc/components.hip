@@ -15,38 +15,68 @@ __global__ void fill_i32_kernel(int* p, int n, int v) {
 // Pixels with any flag bit go to the FP64 list (order is irrelevant: pixels are
 // independent and each writes only its own outputs); with a row-L table, to the list of
 // the bucket of their row's L (list b at list + b * lstride, its length in count[b]), so
-// each bucket runs on the lane shape of its L.  Each lane tests four pixels (one 32-bit
-// load where the four are in range), and each wave reserves its entries with one atomic
-// per bucket, ranked by ballot.
+// each bucket runs on the lane shape of its L.  Each thread tests sixteen pixels (one
+// 16-byte load where they are in range and aligned), and per bucket a block-wide prefix
+// of the threads' counts, in pixel order, and one atomic per block reserve the block's
+// entries: neighbouring pixels stay neighbours in the list (one atomic per wave and
+// bucket, over four pixels per lane, had the whole grid contend for the counters: 0.20 ms
+// for the expanding-aperture frame's 1,047,552 flags).
 __global__ __launch_bounds__(256) void compact_flags_kernel(const uint8_t* flags, int n, int* list, int* count,
                                                             const int* rowL, int nzp, int Y, RowBuckets bk, int nb,
                                                             int lstride) {
-  const int i0 = 4 * (blockIdx.x * 256 + threadIdx.x);
-  uint32_t f4 = 0;
-  if (i0 + 3 < n && ((uintptr_t)flags & 3) == 0) {
-    f4 = *reinterpret_cast<const uint32_t*>(flags + i0);
+  __shared__ int wsum[4];
+  __shared__ int bbase;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int i0 = 16 * (blockIdx.x * 256 + t);
+  uint32_t f[4] = {0u, 0u, 0u, 0u};
+  if (i0 + 15 < n && ((uintptr_t)(flags + i0) & 15) == 0) {
+    const uint4 v = *reinterpret_cast<const uint4*>(flags + i0);
+    f[0] = v.x;
+    f[1] = v.y;
+    f[2] = v.z;
+    f[3] = v.w;
   } else {
-    for (int u = 0; u < 4; ++u)
-      if (i0 + u < n) f4 |= (uint32_t)flags[i0 + u] << (8 * u);
+    for (int u = 0; u < 16; ++u)
+      if (i0 + u < n) f[u >> 2] |= (uint32_t)flags[i0 + u] << (8 * (u & 3));
   }
-  const int lane = threadIdx.x & 63;
-  for (int u = 0; u < 4; ++u) {
-    const bool hit = ((f4 >> (8 * u)) & 0xffu) != 0;
-    int b = 0;
-    if (rowL && hit) {
-      int L = rowL[(i0 + u) % nzp + Y];
-      L = L < 0 ? 0 : L > 64 ? 64 : L;
-      b = bk.b[L];
+  // bucket + 1 of each flagged pixel, four bits each (0: not flagged)
+  unsigned long long code = 0;
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    if ((f[u >> 2] >> (8 * (u & 3))) & 0xffu) {
+      int b = 0;
+      if (rowL) {
+        int L = rowL[(i0 + u) % nzp + Y];
+        L = L < 0 ? 0 : L > 64 ? 64 : L;
+        b = bk.b[L];
+      }
+      code |= (unsigned long long)(b + 1) << (4 * u);
     }
-    for (int bb = 0; bb < nb; ++bb) {
-      const bool mine = hit && b == bb;
-      const unsigned long long m = __ballot(mine);
-      if (m == 0) continue;
-      int base = 0;
-      if (lane == __ffsll((long long)m) - 1) base = atomicAdd(count + bb, __popcll(m));
-      base = __shfl(base, __ffsll((long long)m) - 1);
-      if (mine) list[(size_t)bb * lstride + base + __popcll(m & ((1ull << lane) - 1ull))] = i0 + u;
+  }
+  for (int bb = 0; bb < nb; ++bb) {
+    int c = 0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) c += ((code >> (4 * u)) & 15u) == (unsigned long long)(bb + 1);
+    int inc = c;   // inclusive prefix over the wave's lanes
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(inc, o);
+      if (lane >= o) inc += v;
     }
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    int wbase = 0, tot = 0;
+    for (int w = 0; w < 4; ++w) {
+      wbase += w < wv ? wsum[w] : 0;
+      tot += wsum[w];
+    }
+    if (t == 0) bbase = tot ? atomicAdd(count + bb, tot) : 0;
+    __syncthreads();
+    int pos = bbase + wbase + inc - c;
+    if (c)
+      for (int u = 0; u < 16; ++u)
+        if (((code >> (4 * u)) & 15u) == (unsigned long long)(bb + 1)) list[(size_t)bb * lstride + pos++] = i0 + u;
+    __syncthreads();   // wsum and bbase are reused by the next bucket
   }
 }
 
@@ -119,7 +149,7 @@ hipError_t compact_flags_rows(const uint8_t* flags, int n, int* list, int* count
       if (bk.b[L] >= nb) return hipErrorInvalidValue;
   hipError_t e = fill_i32(count, nb, 0, s);
   if (e != hipSuccess || n <= 0) return e;
-  hipLaunchKernelGGL(compact_flags_kernel, dim3((n + 1023) / 1024), dim3(256), 0, s, flags, n, list, count, rowL,
+  hipLaunchKernelGGL(compact_flags_kernel, dim3((n + 4095) / 4096), dim3(256), 0, s, flags, n, list, count, rowL,
                      nzp, Y, bk, nb, lstride);
   return hipGetLastError();
 }

@@ -524,9 +524,12 @@ def test_autotune_aperture_rows():
 def test_fallback_list_past_the_expected_count():
     """More flagged pixels than the host expects (FP32 breakdowns on rank-deficient windows,
     none flagged by the condition bound): the list entries past the expected count go to
-    the persistent FP64 grid, and every flagged pixel matches the all-FP64 frame, which
-    runs the same FP64 kernel without a list."""
-    M, L, Y, d = 33, 32, 0, 1e-12   # two subarrays: S_L has rank 2 plus a loading of 1e-12
+    the persistent FP64 grid, and every flagged pixel matches the all-FP64 frame.  The list
+    kernel forms the covariance by the FP64 recursion, the whole-frame kernel by direct
+    sums, so the two agree to the window's conditioning: with two subarrays S has rank 2,
+    and the loading d = 1e-8 gives cond(S_L) ~ L/(2d) = 1.6e9 (FP32 pivots drown in
+    rounding, FP64 ones keep ~7 digits)."""
+    M, L, Y, d = 33, 32, 0, 1e-8   # two subarrays: S_L has rank 2 plus a loading of 1e-8
     say("fallback tail", 8, 2048, M, L, Y, d)
     x = uniform_cube(33, 8, 2048, M)
     o, plan = _plan_run(x, d, L, Y, precision="fp32+fallback", cond_thresh=1e30, want=("P", "z", "w", "flags"))
@@ -538,9 +541,9 @@ def test_fallback_list_past_the_expected_count():
     assert np.all((o["flags"][flagged] & 2) != 0)   # breakdowns, not the bound
     ref, _ = _plan_run(x, d, L, Y, precision="fp64", want=("P", "z", "w"))
     assert np.all(np.isfinite(o["P"][flagged]))
-    np.testing.assert_allclose(o["P"][flagged], ref["P"][flagged], rtol=1e-6, atol=0)
-    np.testing.assert_allclose(o["w"][flagged], ref["w"][flagged], rtol=0, atol=1e-6 * np.abs(ref["w"][flagged]).max())
-    np.testing.assert_allclose(o["z"][flagged], ref["z"][flagged], rtol=0, atol=1e-6 * np.abs(ref["z"][flagged]).max())
+    np.testing.assert_allclose(o["P"][flagged], ref["P"][flagged], rtol=1e-5, atol=0)
+    np.testing.assert_allclose(o["w"][flagged], ref["w"][flagged], rtol=0, atol=1e-5 * np.abs(ref["w"][flagged]).max())
+    np.testing.assert_allclose(o["z"][flagged], ref["z"][flagged], rtol=0, atol=1e-5 * np.abs(ref["z"][flagged]).max())
 
 
 def test_rows_tables_and_fp64_fallback():

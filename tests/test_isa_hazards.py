@@ -24,9 +24,13 @@ SOURCES = ["k_f32_g16c2_t5mr.hip", "k_f32_g16c2_t5.hip", "k_f32_g16c2_t9.hip", "
 
 DPP = re.compile(r"^(v_\w+_dpp)\s+(v\d+),\s*-?\|?(v\d+)")
 DST = re.compile(r"^v_\w+\s+(v\d+|v\[\d+:\d+\])")
+SWAP = re.compile(r"^v_permlane(16|32)_swap\w*\s+(v\d+),\s*(v\d+)")
 
 
 def written(ins):
+    s = SWAP.match(ins)
+    if s:   # the lane swaps write both their operands
+        return [s.group(2), s.group(3)]
     m = DST.match(ins)
     if not (m and ins.startswith("v_")):
         return []
@@ -131,6 +135,14 @@ def test_checker_flags_a_planted_hazard():
              "v_mul_f32_e32 v10, v1, v2", "v_add_f32_e32 v11, v1, v2", "v_fmac_f32_dpp v7, -v10, v8 row_newbcast:3"]
     found, checked = hazards(lines)
     assert checked == 3 and [f[0] for f in found] == ["v_mul_f32_e32 v5, v1, v2", "v_mul_f32_e32 v10, v1, v2"]
+
+
+def test_checker_counts_both_operands_of_a_lane_swap():
+    # v_permlane16_swap writes its second operand too; a DPP read of it right after waits
+    lines = ["v_permlane16_swap_b32_e32 v4, v5", "v_fmac_f32_dpp v7, -v5, v8 row_newbcast:3",
+             "v_permlane16_swap_b32_e32 v9, v10", "s_nop 1", "v_fmac_f32_dpp v7, -v10, v8 row_newbcast:3"]
+    found, checked = hazards(lines)
+    assert checked == 2 and [f[0] for f in found] == ["v_permlane16_swap_b32_e32 v4, v5"]
 
 
 def test_checker_follows_loop_back_edges_and_branches():
