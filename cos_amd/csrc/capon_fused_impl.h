@@ -104,16 +104,21 @@ template <typename T, int G, int CPL, int TM> struct OccHint {
                                               : (G == 16 && CPL == 1 ? 2 : 1);
 };
 
+// WPB: waves per block.  4 for the FP32 64-lane kernels with a fixed window bound: the
+// block's waves take four pixels that, when they are consecutive rows of one beam with one
+// aperture, share the staged window rows and their per-row autocorrelations (each row's
+// computed once for the block instead of once per wave that needs it); a block barrier
+// then hands the rows' LDS to the waves' factorisations.  1 otherwise.
 // MG: for a wave whose four pixels are consecutive rows of one beam (L <= 32), 1 builds
 // their covariances from per-snapshot Grams on the matrix cores (mfma_gram.h) instead of
 // the displacement recursion; 2 keeps the recursion but forms its rank-two corrections on
 // the FP32 matrix cores and leaves only the prefix along the diagonals to the VALU.  The
 // launch's other waves keep the VALU recursion.
-template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0, bool REF = false, int MG = 0>
+template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0, bool REF = false, int MG = 0, int WPB = 1>
 __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, const int GREG,
                                             const int gid0, const int npix) {
   using Gp = Grp<G>;
-  constexpr int NG = Gp::NG, LMAX = G * CPL, NS = G * CPL * (CPL + 1) / 2;
+  constexpr int NG = Gp::NG * WPB, LMAX = G * CPL, NS = G * CPL * (CPL + 1) / 2;   // NG: pixels per block
   // the reference's optional variants (forward-backward averaging, power-Capon amplitude,
   // unloaded power) are compiled into the runtime-window forward kernels only, so the
   // fixed-window kernels keep their registers
@@ -121,7 +126,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   constexpr int TMR = TM > 0 ? TM : 1;
   extern __shared__ float2 xs_all[];
 
-  const int g = Gp::lane(), q = Gp::group();
+  const int g = Gp::lane(), q = WPB > 1 ? (int)(threadIdx.x >> 6) : Gp::group();
   const int Y = a.Y, TW = 2 * Y + 1, Nzp = a.Nz - 2 * Y;
   const int gid = gid0 + q;
   const bool active = gid < npix;
@@ -163,7 +168,22 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // q .. q+2Y.  Otherwise each group stages its own 2Y+1 rows.
   const int NR = NG + TW - 1;
   bool shared = false, mg = false, mr = false;
-  if constexpr (NG > 1) {
+  if constexpr (NG > 1 && WPB > 1) {
+    // one pixel per wave: the vote goes through LDS
+    __shared__ int bv[WPB][4];
+    if (g == 0) {
+      bv[q][0] = active ? p : -1;
+      bv[q][1] = L;
+      bv[q][2] = M;
+      bv[q][3] = off;
+    }
+    __syncthreads();
+    const int p0 = bv[0][0];
+    bool consec = p0 >= 0 && p0 - (p0 / Nzp) * Nzp + NG <= Nzp;
+    for (int w = 1; w < WPB; ++w)
+      consec = consec && bv[w][0] == p0 + w && bv[w][1] == bv[0][1] && bv[w][2] == bv[0][2] && bv[w][3] == bv[0][3];
+    shared = consec && (NR * MP + NR * LMAX * (int)(sizeof(cplx<T>) / sizeof(float2)) <= NG * GREG);
+  } else if constexpr (NG > 1) {
     const int p0 = __builtin_amdgcn_readfirstlane(p);
     const int L0 = __builtin_amdgcn_readfirstlane(L), M0 = __builtin_amdgcn_readfirstlane(M);
     const int o0 = __builtin_amdgcn_readfirstlane(off);
@@ -182,7 +202,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       pexp = mfma_stage(reinterpret_cast<unsigned*>(xs_all), wrow0, a.Mstride, __builtin_amdgcn_readfirstlane(M), NR,
                         mfma_mpf(__builtin_amdgcn_readfirstlane(M - L + 1)));
   } else if (shared) {
-    stage_rows<8, 2>(xs_all, MP, wrow0, a.Mstride, NR, M, threadIdx.x, 64);
+    stage_rows<8, (WPB > 1 ? 1 : 2)>(xs_all, MP, wrow0, a.Mstride, NR, M, threadIdx.x, 64 * WPB);
   } else {
     // Stage the 2Y+1 snapshots of the window, zero padded to MP elements.
     stage_rows<(G == 16 ? 5 : 8), (G == 16 ? 4 : 2)>(xs, MP, a.x + ((size_t)ix * a.Nz + j) * a.Mstride + off, a.Mstride,
@@ -294,32 +314,33 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     });
     if (shared) {
       // Per-row autocorrelations rho_t(c) = sum_{k<K} x_t[k] conj(x_t[k+c]) for the NR
-      // shared rows, each computed once by the wave (lane task: one row, four lags, a
-      // sliding window of four values), then S[0][c] = sum of the group's 2Y+1 rows.
+      // shared rows, each computed once by the block (lane task: one row, NLAG lags, a
+      // sliding window of NLAG values; two lags with four waves, so every lane has a task),
+      // then S[0][c] = sum of the group's 2Y+1 rows.
       cplx<T>* rho = reinterpret_cast<cplx<T>*>(xs_all + NR * MP);
-      constexpr int NLB = LMAX / 4;
-      for (int task = threadIdx.x; task < NR * NLB; task += 64) {
-        const int t = task / NLB, c0 = 4 * (task - t * NLB);
+      constexpr int NLAG = WPB > 1 ? 2 : 4, NLB = LMAX / NLAG;
+      for (int task = threadIdx.x; task < NR * NLB; task += 64 * WPB) {
+        const int t = task / NLB, c0 = NLAG * (task - t * NLB);
         const float2* xr = xs_all + t * MP;
-        cplx<T> acc[4], wv[4];
-        static_for<0, 4>([&](auto U_) {
+        cplx<T> acc[NLAG], wv[NLAG];
+        static_for<0, NLAG>([&](auto U_) {
           constexpr int U = decltype(U_)::value;
           acc[U] = {T(0), T(0)};
           wv[U] = ldx<T>(xr + c0 + U);
         });
-        // K rounded up to a multiple of 4; the extra terms read zero padding (a = 0)
-        for (int k = 0; k < K; k += 4) {
-          static_for<0, 4>([&](auto V_) {
+        // K rounded up to a multiple of NLAG; the extra terms read zero padding (a = 0)
+        for (int k = 0; k < K; k += NLAG) {
+          static_for<0, NLAG>([&](auto V_) {
             constexpr int V = decltype(V_)::value;
             const cplx<T> u = (k + V < K) ? ldx<T>(xr + k + V) : cplx<T>{T(0), T(0)};
-            static_for<0, 4>([&](auto U_) {
+            static_for<0, NLAG>([&](auto U_) {
               constexpr int U = decltype(U_)::value;
-              add_mul_conj(acc[U], u, wv[(U + V) % 4]);
+              add_mul_conj(acc[U], u, wv[(U + V) % NLAG]);
             });
-            wv[V] = ldx<T>(xr + k + V + c0 + 4);
+            wv[V] = ldx<T>(xr + k + V + c0 + NLAG);
           });
         }
-        static_for<0, 4>([&](auto U_) {
+        static_for<0, NLAG>([&](auto U_) {
           constexpr int U = decltype(U_)::value;
           rho[t * LMAX + c0 + U] = acc[U];
         });
@@ -627,6 +648,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   if (a.cov_only) return;
 
   if constexpr (!OPTS) subarray_means();
+  // every wave of the block has read the shared rows: the factorisations may overwrite them
+  if constexpr (WPB > 1) __syncthreads();
 
   // ---------------- Cholesky U^H U = S_L, fused forward solve U^H v = a ----------------
   // LDS region of the group (free now: every read of the snapshots is complete), in
@@ -1331,15 +1354,26 @@ __device__ __noinline__ void capon_group_call(const CaponLaunch& a, const int MP
   capon_group<T, G, CPL, TM, DIRECT, BWD, REF>(a, MP, GREG, gid0, npix);
 }
 
+// Waves per block (see capon_group): four for the FP32 64-lane forward kernels with a fixed
+// window bound, one otherwise.
+template <typename T, int G, int TM, bool PERSIST, int BWD, bool REF> struct Wpb {
+  static constexpr int value = (std::is_same<T, float>::value && G == 64 && TM > 0 && !PERSIST && !BWD && !REF) ? 4 : 1;
+};
+
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false,
           int MG = 0>
-__global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
-                          amdgpu_waves_per_eu(BWD ? 1 : (REF && G == 64) ? 2 : OccHint<T, G, CPL, TM>::value)))
+__global__ __attribute__((amdgpu_flat_work_group_size(64 * Wpb<T, G, TM, PERSIST, BWD, REF>::value,
+                                                      64 * Wpb<T, G, TM, PERSIST, BWD, REF>::value),
+                          amdgpu_waves_per_eu(BWD ? 1
+                                              : ((REF && G == 64) || Wpb<T, G, TM, PERSIST, BWD, REF>::value > 1)
+                                                  ? 2
+                                                  : OccHint<T, G, CPL, TM>::value)))
 void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
   // PERSIST: grid-stride over groups of NG pixels, so a launch over a device-side pixel
   // list (the FP64 fallback) can use a bounded grid; otherwise one block per group (a
   // loop costs the full-frame kernels registers).
-  constexpr int NGk = Grp<G>::NG;
+  constexpr int WPBk = Wpb<T, G, TM, PERSIST, BWD, REF>::value;
+  constexpr int NGk = Grp<G>::NG * WPBk;
   const int npix = a.pcount ? min(*a.pcount - a.pbase, a.npix) : a.rcount ? min(a.Nx * *a.rcount, a.npix) : a.npix;
   if constexpr (PERSIST) {
     for (int gid0 = blockIdx.x * NGk; gid0 < npix; gid0 += gridDim.x * NGk) {
@@ -1362,7 +1396,7 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
     if (b >= B) return;
     const unsigned lb = b < B8 ? (b % NXCD) * (B8 / NXCD) + b / NXCD : b;
     const int gid0 = (int)lb * NGk;
-    if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT, BWD, REF, MG>(a, MP, GREG, gid0, npix);
+    if (gid0 < npix) capon_group<T, G, CPL, TM, DIRECT, BWD, REF, MG, WPBk>(a, MP, GREG, gid0, npix);
   }
 }
 
@@ -1370,7 +1404,8 @@ void capon_kernel(const CaponLaunch a, const int MP, const int GREG) {
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false,
           int MG = 0>
 hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
-  constexpr int NG = 64 / G, LMAX = G * CPL;
+  constexpr int WPB = Wpb<T, G, TM, PERSIST, BWD, REF>::value;
+  constexpr int NG = 64 / G * WPB, LMAX = G * CPL;   // pixels per block
   const int MP = ((Mmax + LMAX + 4 + 3) / 4) * 4;   // zero padding covers every lag read
   const int TW = 2 * a.Y + 1;
   // per-group LDS region: the window's snapshots, later the packed upper U + dump + pad,
@@ -1386,13 +1421,13 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
     GREG = GREG > need ? GREG : need;
   }
   const size_t lds = (size_t)NG * GREG * sizeof(float2);
-  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  if (lds > 160 * 1024) return hipErrorInvalidValue;   // gfx950: 160 KB of LDS per workgroup
   if (!a.x) return hipErrorInvalidValue;
   int blocks = (a.npix + NG - 1) / NG;
   if (PERSIST && blocks > 2048) blocks = 2048;   // list length known only on the device
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST, BWD, REF, MG>), dim3(blocks), dim3(64), lds, s, a, MP,
-                     GREG);
+  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST, BWD, REF, MG>), dim3(blocks), dim3(64 * WPB), lds, s,
+                     a, MP, GREG);
   return hipGetLastError();
 }
 
