@@ -61,8 +61,13 @@ static hipError_t f64_launch(const CaponLaunch& a, int Lmax, int Mmax, bool list
       return list ? launch_one<double, 16, 2, 0, false, true, false>(a, Mmax, s) : launch_one<double, 16, 2, 0, false, false, false>(a, Mmax, s);
     return list ? launch_one<double, 16, 2, 0, true, true, false>(a, Mmax, s) : launch_one<double, 16, 2, 0, true, false, false>(a, Mmax, s);
   }
-  if (Lmax <= 64)   // covariance by the FP64 recursion, the columns in registers
+  if (Lmax <= 64) {   // covariance by the FP64 recursion, the columns in registers
+    // the fallback list with windows of up to five rows: the fixed-window kernel (alpha and
+    // beta of the recursion held in registers instead of reloaded per window row)
+    if (a.plist && 2 * a.Y + 1 <= 5 && !(a.fb || a.zmode || a.pmode))
+      return list ? launch_one<double, 64, 1, 5, false, true, false>(a, Mmax, s) : launch_one<double, 64, 1, 5, false, false, false>(a, Mmax, s);
     return list ? launch_one<double, 64, 1, 0, false, true, false>(a, Mmax, s) : launch_one<double, 64, 1, 0, false, false, false>(a, Mmax, s);
+  }
   if (why) *why = "L > 64 is not supported by the FP64 kernel";
   return hipErrorInvalidValue;
 }

@@ -488,11 +488,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         }
       });
       auto corr = [&](int t, const cplx<T>* alt, const cplx<T>* bet) {
-        // one pixel per wave with the window in registers: u1 = x_t[K-1+R] and
+        // one pixel per wave with the window in registers (FP32): u1 = x_t[K-1+R] and
         // u0 = x_t[R-1] are alpha_t and beta_t of lane R (read into scalars, no LDS
         // round trip; every lane's read of the row would be the same address)
         cplx<T> u1, u0;
-        if constexpr (G == 64 && TM > 0) {
+        if constexpr (G == 64 && TM > 0 && std::is_same<T, float>::value) {
           u1 = bcastc<G, R>(alt[0]);
           u0 = bcastc<G, R>(bet[0]);
         } else {
