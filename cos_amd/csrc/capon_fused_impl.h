@@ -471,6 +471,66 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         });
       });
     }
+    if constexpr (G == 64 && CPL == 1 && TM > 0 && std::is_same<T, float>::value) {
+      // One pixel per wave (FP32): the rank-two sums D = sum_t alpha_t alpha_t^H - beta_t
+      // beta_t^H of all 64 rows on the matrix cores, as the three upper 32 x 32 tiles
+      // (0,0), (0,1), (1,1) on v_mfma_f32_32x32x2_f32 (rank_pass's operand forms, see
+      // mfma_gram.h), then the prefix S[r][c] = S[r-1][c-1] + D[r][c] along the diagonals.
+      // Lane c holds alpha_t[c] and beta_t[c]; a half-wave swap of (re, im) gives the
+      // operands of both row blocks at once (rows 32P + l: re in lane l, im in lane 32 + l).
+      f32x16 d00r = {}, d00i = {}, d01r = {}, d01i = {}, d11r = {}, d11i = {};
+      auto swp = [](float x, float y, float& p0, float& p1) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+        p0 = __uint_as_float(r[0]);
+        p1 = __uint_as_float(r[1]);
+      };
+      static_for<0, TM>([&](auto t_) {
+        constexpr int t = decltype(t_)::value;
+        const cplx<T> av = al[t][0];
+        const cplx<T> bv = g > 0 ? be[t][0] : cplx<T>{T(0), T(0)};   // beta_t[0] = 0
+        float ar0, ar1, ai0, ai1, br0, br1, bi0, bi1;
+        swp(av.x, av.y, ar0, ar1);     // (re | im): rows 0..31, 32..63
+        swp(av.y, -av.x, ai0, ai1);    // (im | -re)
+        swp(bv.x, bv.y, br0, br1);
+        swp(-bv.y, bv.x, bi0, bi1);    // (-im | re)
+        d00r = __builtin_amdgcn_mfma_f32_32x32x2f32(ar0, ar0, d00r, 0, 0, 0);
+        d01r = __builtin_amdgcn_mfma_f32_32x32x2f32(ar0, ar1, d01r, 0, 0, 0);
+        d11r = __builtin_amdgcn_mfma_f32_32x32x2f32(ar1, ar1, d11r, 0, 0, 0);
+        d00i = __builtin_amdgcn_mfma_f32_32x32x2f32(ai0, ar0, d00i, 0, 0, 0);
+        d01i = __builtin_amdgcn_mfma_f32_32x32x2f32(ai0, ar1, d01i, 0, 0, 0);
+        d11i = __builtin_amdgcn_mfma_f32_32x32x2f32(ai1, ar1, d11i, 0, 0, 0);
+        d00r = __builtin_amdgcn_mfma_f32_32x32x2f32(br0, -br0, d00r, 0, 0, 0);
+        d01r = __builtin_amdgcn_mfma_f32_32x32x2f32(br0, -br1, d01r, 0, 0, 0);
+        d11r = __builtin_amdgcn_mfma_f32_32x32x2f32(br1, -br1, d11r, 0, 0, 0);
+        d00i = __builtin_amdgcn_mfma_f32_32x32x2f32(bi0, br0, d00i, 0, 0, 0);
+        d01i = __builtin_amdgcn_mfma_f32_32x32x2f32(bi0, br1, d01i, 0, 0, 0);
+        d11i = __builtin_amdgcn_mfma_f32_32x32x2f32(bi1, br1, d11i, 0, 0, 0);
+      });
+      // accumulator element v of lane l: column 32Q + l % 32, row 32P + 8(v/4) + 4(l/32) + v%4.
+      // Swapping the halves of tiles (0,0) and (0,1) gives every lane rows r0 and r0 + 4 of
+      // its own column; tile (1,1) swapped with itself gives lanes 32.. rows 32 + r0 (their
+      // own elements hold 32 + r0 + 4).  Lanes below 32 receive rows 32.. of columns below
+      // 32, under the diagonal, which nothing reads.  Row 0 keeps the direct first row.
+      static_for<0, 16>([&](auto V_) {
+        constexpr int v = decltype(V_)::value;
+        constexpr int r0 = 8 * (v / 4) + v % 4, r1 = r0 + 4;
+        float a0, a1, b0, b1;
+        swp(d00r[v], d01r[v], a0, a1);
+        swp(d00i[v], d01i[v], b0, b1);
+        if constexpr (r0 > 0) A[r0] = {a0, b0};
+        A[r1] = {a1, b1};
+        swp(d11r[v], d11r[v], a0, a1);
+        swp(d11i[v], d11i[v], b0, b1);
+        A[32 + r0] = {a0, b0};
+        A[32 + r1] = {d11r[v], d11i[v]};
+      });
+      static_for<1, LMAX>([&](auto R_) {
+        constexpr int R = decltype(R_)::value;
+        if (R >= Lw) return;
+        A[R].x += Gp::ror1(A[R - 1].x);
+        A[R].y += Gp::ror1(A[R - 1].y);
+      });
+    } else
     static_for<1, LMAX>([&](auto R_) {
     constexpr int R = decltype(R_)::value;
       if (SKIP && R >= Lw) return;
