@@ -216,6 +216,27 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
 
   cplx<T> A[NS];
 
+  // Rows R0.. of the lane's columns g and 16 + g (two columns per lane) from a window in the
+  // 32 x 32 buffer (window_store's column order keeps the two side by side: one 16-byte read
+  // per row for both slots)
+  auto window_take = [&](auto R0_, const float2* Tb) {
+    constexpr int R0 = decltype(R0_)::value;
+    if constexpr (MG != 0) {
+      static_assert(G == 16 && CPL == 2, "the matrix-core covariance paths are for 16 lanes x 2 columns");
+      static_for<R0, 32>([&](auto R_) {
+        constexpr int R = decltype(R_)::value;
+        if constexpr (R < 16) {
+          const float4 v = *reinterpret_cast<const float4*>(Tb + R * 32 + 2 * g);
+          A[slot<G, 0, R>()] = {(T)v.x, (T)v.y};
+          A[slot<G, 1, R>()] = {(T)v.z, (T)v.w};
+        } else {
+          const float2 v = Tb[R * 32 + 2 * g + 1];
+          A[slot<G, 1, R>()] = {(T)v.x, (T)v.y};
+        }
+      });
+    }
+  };
+
   if constexpr (MG == 1) {
     if (mg) {
       // pixel q's window sum of the row Grams into the transfer buffer, then into the
@@ -228,16 +249,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         __syncthreads();   // the previous window has been read
         window_store(Tb, wr, wi);
         __syncthreads();
-        if (q == qq) {
-          static_for<0, CPL>([&](auto S_) {
-            constexpr int S = decltype(S_)::value;
-            static_for<0, G * (S + 1)>([&](auto R_) {
-              constexpr int R = decltype(R_)::value;
-              const float2 v = Tb[R * 32 + cs[S]];
-              A[slot<G, S, R>()] = {(T)v.x, (T)v.y};
-            });
-          });
-        }
+        if (q == qq) window_take(ic<0>{}, Tb);
       };
       const f32x16 zero = {};
       f32x16 ur, ui, xr, xi, sr, si;
@@ -397,16 +409,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           __syncthreads();   // the previous window has been read
           window_store(Tb, wr, wi);
           __syncthreads();
-          if (q == qq) {
-            static_for<0, CPL>([&](auto S_) {
-              constexpr int S = decltype(S_)::value;
-              static_for<1, G * (S + 1)>([&](auto R_) {
-                constexpr int R = decltype(R_)::value;
-                const float2 v = Tb[R * 32 + cs[S]];
-                A[slot<G, S, R>()] = {(T)v.x, (T)v.y};
-              });
-            });
-          }
+          if (q == qq) window_take(ic<1>{}, Tb);
         };
         const f32x16 zero = {};
         f32x16 ur, ui, xr, xi, sr, si;

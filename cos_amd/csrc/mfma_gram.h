@@ -225,13 +225,18 @@ __host__ __device__ constexpr int nlive(int k) {
   return n;
 }
 
-// The window (real sr, imaginary si, MFMA layout) into the 32 x 32 buffer Tb[i][j].
+// The window (real sr, imaginary si, MFMA layout) into the 32 x 32 buffer, row i at
+// Tb[32 i ..], column j at position 2 (j % 16) + j / 16 of the row: the columns g and
+// 16 + g that one lane of a 16-lane group owns sit side by side (one 16-byte read), and
+// both the stores (a permutation of each row) and a group's reads (256 contiguous bytes)
+// are free of bank conflicts.
 __device__ __forceinline__ void window_store(float2* Tb, const f32x16& sr, const f32x16& si) {
   const int c = threadIdx.x & 31, h = (threadIdx.x >> 5) & 1;
+  const int pc = ((c & 15) << 1) | (c >> 4);
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
     const int i = 8 * (v / 4) + 4 * h + (v % 4);
-    Tb[i * 32 + c] = make_float2(sr[v], si[v]);
+    Tb[i * 32 + pc] = make_float2(sr[v], si[v]);
   }
 }
 
