@@ -195,6 +195,17 @@ cosk::CaponLaunch base_launch(const cos_capon_desc& ds, const void* x) {
   a.cond_thresh = ds.cond_thresh > 0 ? ds.cond_thresh : 1.0e6f;
   a.steer_sin = ds.steer_sin;
   a.nsteer = ds.nsteer;
+  // a frame with fewer pixel groups than the GPU holds waves splits its angles over the
+  // grid's second dimension (each block factorises again: one factorisation costs about
+  // what one refined angle does at these sizes); ~4,096 blocks in all
+  if (ds.nsteer > 1) {
+    const long groups = ((long)a.npix + 3) / 4;
+    const int want = (int)std::min<long>(ds.nsteer, (4096 + groups - 1) / groups);
+    if (want > 1) {
+      const int chunk = (ds.nsteer + want - 1) / want;
+      a.steer_chunk = chunk;
+    }
+  }
   a.fb = ds.fb_avg ? 1 : 0;
   a.zmode = ds.z_mode;
   a.pmode = ds.p_mode;

@@ -1008,7 +1008,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     });
   };
 
-  for (int ia = 0; ia < nrhs; ++ia) {
+  // this block's angles: all of them, or its chunk (grid dimension y)
+  const int chunk = steered && a.steer_chunk > 0 ? a.steer_chunk : nrhs;
+  const int ia0 = (int)blockIdx.y * chunk, ia1 = min(nrhs, ia0 + chunk);
+  for (int ia = ia0; ia < ia1; ++ia) {
     const T sn = steered ? (T)a.steer_sin[ia] : T(0);
     cplx<T> gcur[CPL];
     static_for<0, CPL>([&](auto S_) {
@@ -1489,8 +1492,9 @@ hipError_t launch_one(const CaponLaunch& a, int Mmax, hipStream_t s) {
   int blocks = (a.npix + NG - 1) / NG;
   if (PERSIST && blocks > 2048) blocks = 2048;   // list length known only on the device
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST, BWD, REF, MG>), dim3(blocks), dim3(64 * WPB), lds, s,
-                     a, MP, GREG);
+  const int nchunk = a.nsteer > 0 && a.steer_chunk > 0 ? (a.nsteer + a.steer_chunk - 1) / a.steer_chunk : 1;
+  hipLaunchKernelGGL((capon_kernel<T, G, CPL, TM, DIRECT, PERSIST, BWD, REF, MG>), dim3(blocks, nchunk), dim3(64 * WPB),
+                     lds, s, a, MP, GREG);
   return hipGetLastError();
 }
 

@@ -45,6 +45,9 @@ struct CaponLaunch {
   // steering (optional): nsteer sines; outputs P/z are [p][nsteer], w is [p][nsteer][wstride]
   const float* steer_sin;
   int nsteer;
+  // with steering, angles per block (0: all): the grid's second dimension takes the chunks,
+  // each block factorising its pixels and solving for angles blockIdx.y * steer_chunk ..
+  int steer_chunk;
   // options: forward-backward averaging, power-Capon amplitude, unloaded power
   int fb, zmode, pmode;
   // optional loaded covariance per pixel in the reference R layout (mem[r*L+c] =
