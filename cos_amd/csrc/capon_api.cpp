@@ -868,7 +868,7 @@ int cos_lca_run(const cos_cf32* Z, int Nx, int Ny, int Nw, const float* wrf, int
                                           "(the reference writes row Y_AVG outside the image)");
       if (Nw > 256) return fail(hipErrorInvalidValue, "reference mode: the all-ones range filter takes Nw <= 256");
       a.branch = 2;
-      a.rows_per_chunk = std::max(1, std::min(64, 4096 / Nw));
+      a.rows_per_chunk = std::max(1, std::min(1024, 8192 / Nw));   // up to 64 KB of FP64 sums in LDS
     } else {
       if (Ny - 2 * Ym > 0 && Nx - 2 * Xm > 0 && Ny > Nx)
         return fail(hipErrorInvalidValue, "reference mode: this filter needs Ny <= Nx (the reference's general "

@@ -150,10 +150,10 @@ __device__ __forceinline__ double add_prod_f64(double s, double a, double b) {
   return s + a * b;
 }
 
-// s + (a - b), the difference rounded first (lcacx.pyx:171)
-__device__ __forceinline__ double add_diff_f64(double s, double a, double b) {
+// a - b, rounded on its own (the difference of lcacx.pyx:171's s + (a - b))
+__device__ __forceinline__ double diff_f64(double a, double b) {
 #pragma clang fp contract(off)
-  return s + (a - b);
+  return a - b;
 }
 
 // 1 x 1 filter with a nonzero weight: the first window of smallest |Z| at every pixel.
@@ -174,13 +174,19 @@ __global__ __launch_bounds__(256) void lca_ref_argmin_kernel(const LcaLaunch a) 
   if (a.win) a.win[pix] = bw;
 }
 
-constexpr int LCA_REF_LANES_W = 4;   // windows per lane of the sliding kernel (Nw <= 256)
-
-// All-ones range filter: one wave per beam x, lane t keeps the running sums of windows
-// t, t+64, ...; each chunk of sliding rows stages its sums in LDS [rows][Nw] and the lanes
-// then take one row each for the first minimum over w.
-__global__ __launch_bounds__(64) void lca_ref_sliding_kernel(const LcaLaunch a) {
+// All-ones range filter: one block of 256 threads per beam x; thread t < Nw keeps the
+// running sum of window t in FP64, in the reference's order: s <- s + (a - b) row by row,
+// the difference rounded first (lcacx.pyx:168-171).  Per chunk of sliding rows the block
+// forms every row's differences |Z(y+Ym)|^2 - |Z(y-Ym-1)|^2 in parallel into LDS
+// [rows][Nw] (each thread's loads all in flight before the first is used), the window
+// threads add them in row order (32 rows' differences read ahead) and write the sums back
+// in place, and the block takes the rows for the first minimum over w, the gathers of the
+// chosen Z again all in flight.  (One wave per beam that waited on each row's loads took
+// 0.53 ms for 256 x 2,044 rows.)
+constexpr int LCA_REF_THREADS = 256;
+__global__ __launch_bounds__(LCA_REF_THREADS) void lca_ref_sliding_kernel(const LcaLaunch a) {
   extern __shared__ double sb[];
+  constexpr int NT = LCA_REF_THREADS;
   const int x = blockIdx.x, t = threadIdx.x;
   const int YA = a.wy, Ym = YA / 2, Ny = a.Ny, Nw = a.Nw, CH = a.rows_per_chunk;
   const float2* Zx = a.Z + (size_t)x * Ny * Nw;
@@ -188,48 +194,76 @@ __global__ __launch_bounds__(64) void lca_ref_sliding_kernel(const LcaLaunch a) 
   const int ylo = 1 + Ym, yhi = Ny - Ym;   // rows of the sliding loop
   // rows it does not write: 0 and window 0, except row Y_AVG when it lies past them,
   // which keeps the first segment's window 0 (Y_AVG >= ylo always)
-  for (int y = t; y < Ny; y += 64) {
+  for (int y = t; y < Ny; y += NT) {
     if (y >= ylo && y < yhi) continue;
     const float2 v = y == YA ? Zx[(size_t)YA * Nw] : make_float2(0.f, 0.f);
     if (a.img) a.img[ob + y] = v;
     if (a.win) a.win[ob + y] = 0;
   }
-  double s[LCA_REF_LANES_W];
-#pragma unroll
-  for (int i = 0; i < LCA_REF_LANES_W; ++i) s[i] = 0.0;
-  for (int y = 0; y < YA; ++y) {
-#pragma unroll
-    for (int i = 0; i < LCA_REF_LANES_W; ++i) {
-      const int w = t + 64 * i;
-      if (w < Nw) s[i] = s[i] + sq_f64(Zx[(size_t)y * Nw + w]);
-    }
-  }
+  double s = 0.0;   // window t's running sum (t < Nw)
+  if (t < Nw)
+    for (int y = 0; y < YA; ++y) s = s + sq_f64(Zx[(size_t)y * Nw + t]);
   for (int y0 = ylo; y0 < yhi; y0 += CH) {
-    const int nr = min(CH, yhi - y0);
-    for (int r = 0; r < nr; ++r) {
-      const int y = y0 + r;
+    const int nr = min(CH, yhi - y0), ne = nr * Nw;
+    for (int e0 = t; e0 < ne; e0 += NT * 16) {
+      float2 za[16], zb[16];
 #pragma unroll
-      for (int i = 0; i < LCA_REF_LANES_W; ++i) {
-        const int w = t + 64 * i;
-        if (w < Nw) {
-          s[i] = add_diff_f64(s[i], sq_f64(Zx[(size_t)(y + Ym) * Nw + w]), sq_f64(Zx[(size_t)(y - Ym - 1) * Nw + w]));
-          sb[r * Nw + w] = s[i];
-        }
+      for (int u = 0; u < 16; ++u) {
+        const int e = e0 + NT * u, r = e / Nw, w = e - r * Nw, y = y0 + r;
+        za[u] = e < ne ? Zx[(size_t)(y + Ym) * Nw + w] : make_float2(0.f, 0.f);
+        zb[u] = e < ne ? Zx[(size_t)(y - Ym - 1) * Nw + w] : make_float2(0.f, 0.f);
       }
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (e0 + NT * u < ne) sb[e0 + NT * u] = diff_f64(sq_f64(za[u]), sq_f64(zb[u]));
     }
     __syncthreads();
-    for (int r = t; r < nr; r += 64) {
-      const double* row = sb + r * Nw;
-      double best = row[0];
-      int bw = 0;
-      for (int w = 1; w < Nw; ++w)
-        if (row[w] < best) {
-          best = row[w];
-          bw = w;
+    if (t < Nw) {
+      double acc = s;
+      for (int r0 = 0; r0 < nr; r0 += 32) {
+        double d[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) d[u] = r0 + u < nr ? sb[(r0 + u) * Nw + t] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 32; ++u)
+          if (r0 + u < nr) {
+            acc = acc + d[u];
+            sb[(r0 + u) * Nw + t] = acc;
+          }
+      }
+      s = acc;
+    }
+    __syncthreads();
+    for (int r0 = t; r0 < nr; r0 += NT * 4) {
+      int bw[4];
+      float2 zv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = r0 + NT * u;
+        bw[u] = 0;
+        if (r < nr) {
+          const double* row = sb + r * Nw;
+          double best = row[0];
+          for (int w = 1; w < Nw; ++w)
+            if (row[w] < best) {
+              best = row[w];
+              bw[u] = w;
+            }
         }
-      const int y = y0 + r;
-      if (a.img) a.img[ob + y] = Zx[(size_t)y * Nw + bw];
-      if (a.win) a.win[ob + y] = bw;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = r0 + NT * u;
+        zv[u] = r < nr ? Zx[(size_t)(y0 + r) * Nw + bw[u]] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = r0 + NT * u;
+        if (r < nr) {
+          if (a.img) a.img[ob + y0 + r] = zv[u];
+          if (a.win) a.win[ob + y0 + r] = bw[u];
+        }
+      }
     }
     __syncthreads();
   }
@@ -277,10 +311,10 @@ hipError_t launch_lca(const LcaLaunch& a, hipStream_t s) {
     if (a.branch == 1) {
       hipLaunchKernelGGL(lca_ref_argmin_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
     } else if (a.branch == 2) {
-      if (a.Nw > 64 * LCA_REF_LANES_W || a.rows_per_chunk < 1 || a.Ny <= a.wy) return hipErrorInvalidValue;
+      if (a.Nw > LCA_REF_THREADS || a.rows_per_chunk < 1 || a.Ny <= a.wy) return hipErrorInvalidValue;
       const size_t lds = (size_t)a.rows_per_chunk * a.Nw * sizeof(double);
       if (lds > 64 * 1024) return hipErrorInvalidValue;
-      hipLaunchKernelGGL(lca_ref_sliding_kernel, dim3(a.Nx), dim3(64), lds, s, a);
+      hipLaunchKernelGGL(lca_ref_sliding_kernel, dim3(a.Nx), dim3(LCA_REF_THREADS), lds, s, a);
     } else if (a.branch == 3) {
       hipLaunchKernelGGL(lca_ref_general_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
     } else {
