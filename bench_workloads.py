@@ -143,6 +143,8 @@ def aperture(args):
     line["mfma_covariance"] = {"ms_per_frame": round(ms_mf, 3), "vs_all_fp64_P_rel_err_max": float(em.max()),
                                "vs_all_fp64_P_rel_err_mean": float(em.mean()),
                                "path": "BF16 hi/lo MFMA covariance per row run -> R in HBM -> FP32 solves"}
+    if args.no_cpu:
+        return line
     # CPU leg: the oracle's C FP64 port over every output row of 8 beams, one call per run of
     # rows with the same (L, M, first element, d); pixels/s of that subset
     from oracle import c_capon
@@ -203,6 +205,8 @@ def beamspace(args):
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_pixel": bpx,
                         "fp32_tflops": round(fpx * npx / (ms * 1e-3) / 1e12, 2), "flop_per_pixel": fpx}
     line["accuracy"] = {"P_rel_err_max_128px": float(e.max()), "w_rel_err_max_128px": float(ew.max())}
+    if args.no_cpu:
+        return line
     # CPU leg: the numpy FP64 restatement (sliding DFT, Nb x Nb Gram, Cholesky), vectorised
     # over pixels, on 2 beams x 2048 rows
     xc = x[:2, :2048 + 2 * Y].cpu().numpy().astype(np.complex128)
@@ -253,6 +257,8 @@ def steering(args):
     line["accuracy"] = {"P_rel_err_max": float(e.max()), "w_rel_err_max": float(ew.max())}
     line["unrefined"] = {"ms": round(ms0, 4), "powers_per_s": npow / (ms0 * 1e-3), "P_rel_err_max": float(e0.max()),
                          "w_rel_err_max": float(ew0.max())}
+    if args.no_cpu:
+        return line
     # CPU leg: the reference's way to steer, one drop-in call per angle on the cube
     # phase-rotated by exp(-j pi m sin theta), through the oracle's C FP64 port; all 128 angles
     from oracle import c_capon
@@ -274,7 +280,8 @@ def shard(args):
     import torch
     from cos_amd import CaponPlan, synth
     G, Nx, Nz, M, L, Y, d = 8, 256, 2048, 64, 32, 2, 0.01
-    x = torch.from_numpy(synth.config2(Nx, Nz, M, seed=2)[: Nx // G].copy()).cuda()
+    xn = synth.config2(Nx, Nz, M, seed=2)[: Nx // G].copy()
+    x = torch.from_numpy(xn).cuda()
     plan = CaponPlan()
     npx = (Nx // G) * (Nz - 2 * Y)
     out = {"P": torch.empty(npx, device="cuda"), "z": torch.empty(npx, dtype=torch.complex64, device="cuda"),
@@ -289,6 +296,18 @@ def shard(args):
                   "L": L, "Yavg": Y, "pixels": npx})
     line["note"] = ("the shard needs no halo or exchange (pixels are independent along beams); "
                     "G GPUs would each run this launch on their own beams")
+    if not args.no_cpu:   # the oracle's C FP64 port over the whole shard, all host threads
+        from oracle import c_capon
+        cores, _ = _host()
+        t0, reps = time.perf_counter(), 0
+        while reps < 1 or time.perf_counter() - t0 < 1.0:
+            c_capon(xn, d, L, Y, want_w=True, nthreads=cores)
+            reps += 1
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = _cpu_line(reps * npx / tc, "pixels/s",
+                                         f"the whole shard ({Nx // G} beams x {Nz - 2 * Y} rows) {reps} times, FP64 "
+                                         f"C port, OpenMP over pixels, {tc:.2f} s")
+        line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)
     return line
 
 
@@ -354,6 +373,8 @@ def adjoint(args):
     line["accuracy"] = {"dphi_err_vs_central_differences_max": float(gerr.max()),
                         "dphi_sample": "8 seeded element phases, error over max |dLoss/dphi|",
                         "P_rel_err_max_64px": float(eP.max())}
+    if args.no_cpu:
+        return line
     # CPU leg: the reference has no gradient; the oracle's C FP64 port runs the forward power
     # of all 8 phase-corrected frames (the reference's own work per frame)
     cores, _ = _host()
@@ -397,6 +418,18 @@ def tdib(args):
                  dtype="f64 geometry, f32 data")
     line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_pixel": bpx}
+    if not args.no_cpu:   # the oracle's numpy FP64 restatement (np.interp per hydrophone) on 64 beams
+        from oracle import stages_oracle
+        nb = 64
+        mfn, xrn, Dn = mf.cpu().numpy(), x_r.cpu().numpy(), D.cpu().numpy()
+        xan, yan = xarr[:nb].cpu().numpy(), yarr[:nb].cpu().numpy()
+        t0 = time.perf_counter()
+        stages_oracle.tdib_cube(mfn, mtaxe, xrn, Dn, 1500.0, 1.0e5, xan, yan)
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = _cpu_line(nb * n_y / tc, "pixels/s",
+                                         f"{nb} of {n_x} beams x {n_y} pixels x {n_h} hydrophones, numpy FP64 "
+                                         f"restatement in {tc:.2f} s", extra={"cores": 1})
+        line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)
     return line
 
 
@@ -430,6 +463,16 @@ def lca(args):
     gbs_c = bpx * npx / (ms_c * 1e-3) / 1e9
     line["centred"] = {"ms_per_frame": round(ms_c, 4), "pixels_per_s": npx / (ms_c * 1e-3),
                        "GB_s": round(gbs_c, 1), "frac": round(gbs_c / PEAK_HBM_GBS, 4)}
+    if not args.no_cpu:   # the oracle's reference-mode restatement (numpy, vectorised over beams)
+        from oracle import stages_oracle
+        Zn, wn = Z.cpu().numpy(), wrf.cpu().numpy()
+        t0 = time.perf_counter()
+        stages_oracle.lca_reference(Zn, wn)
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = _cpu_line(npx / tc, "pixels/s",
+                                         f"the whole {Nx} x {Ny} x {Nw} frame, numpy FP64 restatement of the "
+                                         f"reference's sliding sums in {tc:.2f} s", extra={"cores": 1})
+        line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)
     return line
 
 
