@@ -112,7 +112,8 @@ template <typename T, int G, int CPL, int TM> struct OccHint {
 // MG: for a wave whose four pixels are consecutive rows of one beam (L <= 32), 1 builds
 // their covariances from per-snapshot Grams on the matrix cores (mfma_gram.h) instead of
 // the displacement recursion; 2 keeps the recursion but forms its rank-two corrections on
-// the FP32 matrix cores and leaves only the prefix along the diagonals to the VALU.  The
+// the matrix cores (FP16 hi/lo-split operands, FP32 accumulation, rows staged with a
+// power-of-two scale) and leaves only the prefix along the diagonals to the VALU.  The
 // launch's other waves keep the VALU recursion.
 template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0, bool REF = false, int MG = 0, int WPB = 1>
 __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, const int GREG,
@@ -192,15 +193,19 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     const bool consec = __all(mine);
     shared = consec && (NR * MP + NR * LMAX * (int)(sizeof(cplx<T>) / sizeof(float2)) <= NG * GREG);
     if constexpr (MG == 1) mg = consec && L <= 32 && mfma_lds(M - L + 1, NR) <= NG * GREG;
-    if constexpr (MG == 2) mr = shared && L <= 32 && TW <= TM && NR * MP + NR * LMAX + 1024 <= NG * GREG;
+    if constexpr (MG == 2)
+      mr = shared && L <= 32 && TW <= TM && NR <= 8 && MP <= 128 && NR * MP + NR * LMAX + 1024 <= NG * GREG;
   }
   float2* xs = shared ? xs_all + q * MP : xs_all + q * GREG;
   const float2* const wrow0 = a.x + ((size_t)(p - q) / Nzp * a.Nz + (j - q)) * a.Mstride + off;   // (shared, mg)
-  int pexp = 0;   // with mg: the wave's rows were scaled by 2^pexp, P by 2^(2 pexp)
+  // with mg or mr: the wave's rows were scaled by 2^pexp, so S (and P) by 2^(2 pexp)
+  int pexp = 0;
   if (mg) {
     if constexpr (MG)
       pexp = mfma_stage(reinterpret_cast<unsigned*>(xs_all), wrow0, a.Mstride, __builtin_amdgcn_readfirstlane(M), NR,
                         mfma_mpf(__builtin_amdgcn_readfirstlane(M - L + 1)));
+  } else if (MG == 2 && mr) {
+    pexp = stage_rows_scaled<8, 2>(xs_all, MP, wrow0, a.Mstride, NR, M);
   } else if (shared) {
     stage_rows<8, (WPB > 1 ? 1 : 2)>(xs_all, MP, wrow0, a.Mstride, NR, M, threadIdx.x, 64 * WPB);
   } else {
@@ -417,24 +422,24 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           // four windows of five rows out of eight in ten passes (as the Gram tree):
           // U = E3 + E4, X = U + E2 + E1, S0 = X + E0, S1 = X + E5,
           // Y = U + E5 + E6 (in U), S2 = Y + E2, S3 = Y + E7
-          rank_pass(xs_all, MP, 3, K0, zero, zero, ur, ui);
-          rank_pass(xs_all, MP, 4, K0, ur, ui, ur, ui);
-          rank_pass(xs_all, MP, 2, K0, ur, ui, xr, xi);
-          rank_pass(xs_all, MP, 1, K0, xr, xi, xr, xi);
-          rank_pass(xs_all, MP, 0, K0, xr, xi, sr, si);
+          rank_pass_h(xs_all, MP, 3, K0, zero, zero, ur, ui);
+          rank_pass_h(xs_all, MP, 4, K0, ur, ui, ur, ui);
+          rank_pass_h(xs_all, MP, 2, K0, ur, ui, xr, xi);
+          rank_pass_h(xs_all, MP, 1, K0, xr, xi, xr, xi);
+          rank_pass_h(xs_all, MP, 0, K0, xr, xi, sr, si);
           take(sr, si, 0);
-          rank_pass(xs_all, MP, 5, K0, xr, xi, xr, xi);
+          rank_pass_h(xs_all, MP, 5, K0, xr, xi, xr, xi);
           take(xr, xi, 1);
-          rank_pass(xs_all, MP, 5, K0, ur, ui, ur, ui);
-          rank_pass(xs_all, MP, 6, K0, ur, ui, ur, ui);
-          rank_pass(xs_all, MP, 2, K0, ur, ui, xr, xi);
+          rank_pass_h(xs_all, MP, 5, K0, ur, ui, ur, ui);
+          rank_pass_h(xs_all, MP, 6, K0, ur, ui, ur, ui);
+          rank_pass_h(xs_all, MP, 2, K0, ur, ui, xr, xi);
           take(xr, xi, 2);
-          rank_pass(xs_all, MP, 7, K0, ur, ui, ur, ui);
+          rank_pass_h(xs_all, MP, 7, K0, ur, ui, ur, ui);
           take(ur, ui, 3);
         } else {
           for (int qq = 0; qq < NG; ++qq) {
-            rank_pass(xs_all, MP, qq, K0, zero, zero, sr, si);
-            for (int t = 1; t < TW; ++t) rank_pass(xs_all, MP, qq + t, K0, sr, si, sr, si);
+            rank_pass_h(xs_all, MP, qq, K0, zero, zero, sr, si);
+            for (int t = 1; t < TW; ++t) rank_pass_h(xs_all, MP, qq + t, K0, sr, si, sr, si);
             take(sr, si, qq);
           }
         }
@@ -477,8 +482,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     if constexpr (G == 64 && CPL == 1 && TM > 0 && std::is_same<T, float>::value) {
       // One pixel per wave (FP32): the rank-two sums D = sum_t alpha_t alpha_t^H - beta_t
       // beta_t^H of all 64 rows on the matrix cores, as the three upper 32 x 32 tiles
-      // (0,0), (0,1), (1,1) on v_mfma_f32_32x32x2_f32 (rank_pass's operand forms, see
-      // mfma_gram.h), then the prefix S[r][c] = S[r-1][c-1] + D[r][c] along the diagonals.
+      // (0,0), (0,1), (1,1) on v_mfma_f32_32x32x2_f32 (the operand forms of the rank-two
+      // block in mfma_gram.h), then the prefix S[r][c] = S[r-1][c-1] + D[r][c] along the diagonals.
       // Lane c holds alpha_t[c] and beta_t[c]; a half-wave swap of (re, im) gives the
       // operands of both row blocks at once (rows 32P + l: re in lane l, im in lane 32 + l).
       f32x16 d00r = {}, d00i = {}, d01r = {}, d01i = {}, d11r = {}, d11i = {};
@@ -701,6 +706,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         if (cc < L && R <= cc) {
           cplx<T> v = A[slot<G, S, R>()];
           if (R == cc) v.x += delta;
+          if constexpr (MG == 2) v = {ldexpf(v.x, -2 * pexp), ldexpf(v.y, -2 * pexp)};
           Rp[R * L + cc] = make_float2((float)v.x, (float)-v.y);
           if (R < cc) Rp[cc * L + R] = make_float2((float)v.x, (float)v.y);
         }
@@ -1048,7 +1054,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     const float nanv = __builtin_nanf("");
     if (!REF && active && g == 0 && a.P && (!OPTS || a.pmode == 0)) {
       float pv = (float)(T(1) / (norm * accP));
-      if constexpr (MG == 1) pv = ldexpf(pv, -2 * pexp);   // the solve ran on 2^(2 pexp) S
+      if constexpr (MG != 0) pv = ldexpf(pv, -2 * pexp);   // the solve ran on 2^(2 pexp) S
       a.P[po] = valid ? pv : nanv;
     }
 
@@ -1258,6 +1264,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           pk = Gp::sum(pk);
           zp = {sqrt(pk) / (T)K, T(0)};
         }
+      }
+      if constexpr (MG == 2) {   // g came from the scaled rows (w is scale-free)
+        if (!steered) zp = {ldexpf(zp.x, -pexp), ldexpf(zp.y, -pexp)};
       }
       if (active && g == 0 && a.z)
         a.z[po] = valid ? make_float2((float)zp.x, (float)zp.y) : make_float2(nanv, nanv);

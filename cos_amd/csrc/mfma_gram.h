@@ -141,36 +141,96 @@ __device__ __forceinline__ void gram_pass(const unsigned* F, int t, int MPF, int
   di = im;
 }
 
-// ---- rank-two corrections of the displacement recursion (FP32 matrix cores) ----------
+// ---- rank-two corrections of the displacement recursion (matrix cores) ---------------
 // The recursion S[r][c] = S[r-1][c-1] + D[r][c] (capon_fused_impl.h) adds, per window row t,
 //   E_t[r][c] = alpha_t[r] conj(alpha_t[c]) - beta_t[r] conj(beta_t[c]),
 //   alpha_t[r] = x_t[K-1+r], beta_t[r] = x_t[r-1]
 // (framework/beamformer/capon/RealTimeCapon/buildR_kernel.cu:116-137 forms the same terms
-// element by element).  For a 32 x 32 block, E_t is four real rank-two products, each one
-// v_mfma_f32_32x32x2_f32 (FP32 operands and accumulation, no split):
+// element by element).  For a 32 x 32 block, in real terms,
 //   Re E = [ar ai] [ar ai]^T - [br bi] [br bi]^T
 //   Im E = [ai -ar] [ar ai]^T + [-bi br] [br bi]^T
-// with lane l supplying row (and column) l % 32 and the inner index l / 32.  A window sum
-// of such terms replaces the recursion's VALU work; the prefix along the diagonals stays
-// on the VALU.  (Three 16 x 16 blocks on v_mfma_f32_16x16x4_f32, six independent chains
-// per row instead of two, measured slower: 1.617 against 1.536 ms per frame, the operand
-// selection for four inner indices costing more VALU than the shorter chains saved.)
+// with lane l supplying row (and column) l % 32.  A window sum of such terms replaces the
+// recursion's VALU work; the prefix along the diagonals stays on the VALU.  With FP32
+// operands each product is one v_mfma_f32_32x32x2_f32 (inner index l / 32), 64 cycles
+// during which the SIMD issues nothing else; rank_pass_h puts all four, the operands split
+// hi + lo in FP16, into the inner index of two v_mfma_f32_32x32x16_f16 of 32 cycles each
+// (headline frame: 1.383 -> 1.329 ms, P error against FP64 max 5.1e-6 -> 4.8e-6).  (Three
+// 16 x 16 blocks on v_mfma_f32_16x16x4_f32, six independent chains per row instead of two,
+// measured slower: 1.617 against 1.536 ms per frame, the operand selection for four inner
+// indices costing more VALU than the shorter chains saved.)
 
-// D = C + E_t for the window row t of the staged rows (row stride MP, K subarrays).
-__device__ __forceinline__ void rank_pass(const float2* rows, int MP, int t, int K, const f32x16& cr,
-                                          const f32x16& ci, f32x16& dr, f32x16& di) {
+// Stages the wave's nrows <= RB rows of M elements (src + t sstride, zero padded to MP
+// <= 64 CB) scaled by 2^e, e chosen so that the largest |re| or |im| lies in [2^7, 2^8)
+// (e = 0 for all-zero or non-finite data), and returns e: a power-of-two scale is exact,
+// so everything built from the rows is 2^e (or 2^(2e) for second moments) times the
+// unscaled result, and the FP16 split below keeps its 22 significant bits for every
+// value down to 2^-14 of the largest.  All loads are issued before the reduction.
+template <int RB, int CB>
+__device__ __forceinline__ int stage_rows_scaled(float2* dst, int MP, const float2* src, size_t sstride, int nrows,
+                                                 int M) {
+  const int l = threadIdx.x & 63;
+  float2 v[RB][CB];
+  float mx = 0.f;
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      const int m = l + 64 * c;
+      v[r][c] = (r < nrows && m < M) ? src[(size_t)r * sstride + m] : make_float2(0.f, 0.f);
+      mx = fmaxf(mx, fmaxf(fabsf(v[r][c].x), fabsf(v[r][c].y)));
+    }
+  mx = wave_max(mx);
+  int ex = 0;
+  (void)frexpf(mx, &ex);   // mx < 2^ex
+  const int e = (mx > 0.f && mx <= 3.0e38f) ? 8 - ex : 0;
+  const float sc = ldexpf(1.f, e);
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      const int m = l + 64 * c;
+      if (r < nrows && m < MP) dst[r * MP + m] = make_float2(v[r][c].x * sc, v[r][c].y * sc);
+    }
+  return e;
+}
+
+// One staged (scaled) value as FP16 hi + lo, packed (re, im), and the rotated pairs
+// (im, -re) that the imaginary part's left operand takes: hi = RN(v), lo = RN(v - hi).
+struct SplitC {
+  unsigned h, l, hr, lr;
+};
+__device__ __forceinline__ SplitC split_c(float2 v) {
+  const _Float16 rh = (_Float16)v.x, ih = (_Float16)v.y;
+  const _Float16 rl = (_Float16)(v.x - (float)rh), il = (_Float16)(v.y - (float)ih);
+  return {pack_h2(rh, ih), pack_h2(rl, il), pack_h2(ih, -rh), pack_h2(il, -rl)};
+}
+
+// D = C + E_t for window row t of the staged rows (scaled by stage_rows_scaled; row stride
+// MP, K subarrays) on two v_mfma_f32_32x32x16_f16.  Each operand value is split hi + lo in
+// FP16 and the inner index k = 8 (l / 32) + s carries the split products: lane l supplies
+// row (and column) l % 32 with
+//   A = [a_h b_h a_l b_l]                 (both halves of the wave)
+//   B = [a_h -b_h a_h -b_h]  (l < 32),    [a_l -b_l a_l -b_l]  (l >= 32)
+// (a = alpha_t, b = beta_t, each entry a packed (re, im) pair), so the products are
+// hi.hi + lo.hi in the first half and hi.lo + lo.lo in the second: every term of
+// Re E = ar ar + ai ai - br br - bi bi to about 2^-22 of its size, accumulated in FP32.
+// The imaginary part takes A with every pair rotated to (im, -re) and the same B.
+__device__ __forceinline__ void rank_pass_h(const float2* rows, int MP, int t, int K, const f32x16& cr,
+                                            const f32x16& ci, f32x16& dr, f32x16& di) {
   const int i = threadIdx.x & 31;
   const bool h = (threadIdx.x & 32) != 0;
   const float2* xr = rows + t * MP;
   const float2 al = xr[K - 1 + i];
   const float2 bv = xr[i > 0 ? i - 1 : 0];
   const float2 be = i > 0 ? bv : make_float2(0.f, 0.f);
-  const float a0 = h ? al.y : al.x, b0 = h ? be.y : be.x;
-  const float a1 = h ? -al.x : al.y, b1 = h ? be.x : -be.y;
-  dr = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, a0, cr, 0, 0, 0);
-  di = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, a0, ci, 0, 0, 0);
-  dr = __builtin_amdgcn_mfma_f32_32x32x2f32(b0, -b0, dr, 0, 0, 0);
-  di = __builtin_amdgcn_mfma_f32_32x32x2f32(b1, b0, di, 0, 0, 0);
+  const SplitC sa = split_c(al), sb = split_c(be);
+  constexpr unsigned NEG = 0x80008000u;   // both halves' signs: negation is exact
+  const u32x4 A = {sa.h, sb.h, sa.l, sb.l};
+  const u32x4 Ai = {sa.hr, sb.hr, sa.lr, sb.lr};
+  const unsigned b0 = h ? sa.l : sa.h, b1 = (h ? sb.l : sb.h) ^ NEG;
+  const u32x4 B = {b0, b1, b0, b1};
+  dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(A), as_h8(B), cr, 0, 0, 0);
+  di = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(Ai), as_h8(B), ci, 0, 0, 0);
 }
 
 // ---- 16 x 16 blocks of the Cholesky trailing update (FP32 matrix cores) ----------------
