@@ -125,6 +125,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // fixed-window kernels keep their registers
   constexpr bool OPTS = TM == 0 && !BWD;
   constexpr int TMR = TM > 0 ? TM : 1;
+  // one pixel per wave, FP32, fixed window bound, no refinement: the recursion's rank-two
+  // sums on FP16 hi/lo-split matrix-core products (the refinement kernels keep FP32 ones)
+  constexpr bool HSPLIT = G == 64 && CPL == 1 && TM > 0 && std::is_same<T, float>::value && !REF && !BWD;
   extern __shared__ float2 xs_all[];
 
   const int g = Gp::lane(), q = WPB > 1 ? (int)(threadIdx.x >> 6) : Gp::group();
@@ -492,6 +495,52 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         p0 = __uint_as_float(r[0]);
         p1 = __uint_as_float(r[1]);
       };
+      if constexpr (HSPLIT) {
+        // FP16 hi/lo-split operands on v_mfma_f32_32x32x16_f16 (two of 32 cycles per tile and
+        // row instead of four of 64).  The operands are scaled by 2^pexp, the largest |re|,
+        // |im| of the wave's alpha and beta into [2^7, 2^8), so D is 2^(2 pexp) times the
+        // data's; the first row and the trace are scaled to match below, P scaled back.
+        // Inner index k = 8 (l / 32) + s: half 0 carries hi.hi and lo.lo, half 1 lo.hi and
+        // hi.lo of every (re, im) pair,
+        //   A (rows)    = [a_h b_h a_l b_l | a_l b_l a_h b_h]
+        //   B (columns) = [a_h -b_h a_l -b_l | a_h -b_h a_l -b_l]
+        // (the imaginary part's A with each pair rotated to (im, -re)).  One half-wave swap
+        // of (half-0 value, half-1 value) gives the register for both row (column) blocks:
+        // r[0] for rows 0..31 (lanes 32.. receive lane l - 32's value), r[1] for rows 32..63.
+        float mx = 0.f;
+        static_for<0, TM>([&](auto t_) {
+          constexpr int t = decltype(t_)::value;
+          mx = fmaxf(mx, fmaxf(fmaxf(fabsf(al[t][0].x), fabsf(al[t][0].y)), fmaxf(fabsf(be[t][0].x), fabsf(be[t][0].y))));
+        });
+        mx = wave_max(mx);
+        int ex = 0;
+        (void)frexpf(mx, &ex);   // mx < 2^ex
+        pexp = (mx > 0.f && mx <= 3.0e38f) ? 8 - ex : 0;
+        const float scl = ldexpf(1.f, pexp);
+        auto sw2 = [](unsigned x, unsigned y) { return __builtin_amdgcn_permlane32_swap(x, y, false, false); };
+        constexpr unsigned NEG = 0x80008000u;
+        static_for<0, TM>([&](auto t_) {
+          constexpr int t = decltype(t_)::value;
+          const cplx<T> av = al[t][0];
+          const cplx<T> bv = g > 0 ? be[t][0] : cplx<T>{T(0), T(0)};   // beta_t[0] = 0
+          const SplitC sa = split_c(make_float2(av.x * scl, av.y * scl));
+          const SplitC sb = split_c(make_float2(bv.x * scl, bv.y * scl));
+          const auto x0 = sw2(sa.h, sa.l), x1 = sw2(sb.h, sb.l), x2 = sw2(sa.l, sa.h), x3 = sw2(sb.l, sb.h);
+          const auto y0 = sw2(sa.hr, sa.lr), y1 = sw2(sb.hr, sb.lr), y2 = sw2(sa.lr, sa.hr), y3 = sw2(sb.lr, sb.hr);
+          const unsigned nbh = sb.h ^ NEG, nbl = sb.l ^ NEG;
+          const auto z0 = sw2(sa.h, sa.h), z1 = sw2(nbh, nbh), z2 = sw2(sa.l, sa.l), z3 = sw2(nbl, nbl);
+          const u32x4 A0 = {x0[0], x1[0], x2[0], x3[0]}, A1 = {x0[1], x1[1], x2[1], x3[1]};
+          const u32x4 I0 = {y0[0], y1[0], y2[0], y3[0]}, I1 = {y0[1], y1[1], y2[1], y3[1]};
+          const u32x4 B0 = {z0[0], z1[0], z2[0], z3[0]}, B1 = {z0[1], z1[1], z2[1], z3[1]};
+          d00r = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(A0), as_h8(B0), d00r, 0, 0, 0);
+          d01r = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(A0), as_h8(B1), d01r, 0, 0, 0);
+          d11r = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(A1), as_h8(B1), d11r, 0, 0, 0);
+          d00i = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(I0), as_h8(B0), d00i, 0, 0, 0);
+          d01i = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(I0), as_h8(B1), d01i, 0, 0, 0);
+          d11i = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(I1), as_h8(B1), d11i, 0, 0, 0);
+        });
+        A[0] = {ldexpf(A[0].x, 2 * pexp), ldexpf(A[0].y, 2 * pexp)};
+      } else
       static_for<0, TM>([&](auto t_) {
         constexpr int t = decltype(t_)::value;
         const cplx<T> av = al[t][0];
@@ -622,6 +671,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       }
     }
     trS = Gp::sum(trp);
+    if constexpr (HSPLIT) trS = ldexpf(trS, 2 * pexp);   // S was built 2^(2 pexp) times the data's
   }
   const T delta = from_r ? T(0) : (T)dload * trS / (T)L;   // a.Rin is loaded already
   const T invK = T(1) / (T)K;
@@ -706,7 +756,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         if (cc < L && R <= cc) {
           cplx<T> v = A[slot<G, S, R>()];
           if (R == cc) v.x += delta;
-          if constexpr (MG == 2) v = {ldexpf(v.x, -2 * pexp), ldexpf(v.y, -2 * pexp)};
+          if constexpr (MG == 2 || HSPLIT) v = {ldexpf(v.x, -2 * pexp), ldexpf(v.y, -2 * pexp)};
           Rp[R * L + cc] = make_float2((float)v.x, (float)-v.y);
           if (R < cc) Rp[cc * L + R] = make_float2((float)v.x, (float)v.y);
         }
@@ -1054,7 +1104,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     const float nanv = __builtin_nanf("");
     if (!REF && active && g == 0 && a.P && (!OPTS || a.pmode == 0)) {
       float pv = (float)(T(1) / (norm * accP));
-      if constexpr (MG != 0) pv = ldexpf(pv, -2 * pexp);   // the solve ran on 2^(2 pexp) S
+      if constexpr (MG != 0 || HSPLIT) pv = ldexpf(pv, -2 * pexp);   // the solve ran on 2^(2 pexp) S
       a.P[po] = valid ? pv : nanv;
     }
 
