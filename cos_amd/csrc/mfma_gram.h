@@ -196,13 +196,22 @@ __device__ __forceinline__ int stage_rows_scaled(float2* dst, int MP, const floa
 
 // One staged (scaled) value as FP16 hi + lo, packed (re, im), and the rotated pairs
 // (im, -re) that the imaginary part's left operand takes: hi = RN(v), lo = RN(v - hi).
+// lo is formed as fma(hi, -1, v) rounded once to FP16 (v - hi is exact in FP32, so this
+// is RN(v - hi)); the -1 is hidden from the compiler so that it emits the mixed-precision
+// v_fma_mix{lo,hi}_f16 instead of a conversion back to FP32, a subtraction and a second
+// conversion.
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 struct SplitC {
   unsigned h, l, hr, lr;
 };
 __device__ __forceinline__ SplitC split_c(float2 v) {
-  const _Float16 rh = (_Float16)v.x, ih = (_Float16)v.y;
-  const _Float16 rl = (_Float16)(v.x - (float)rh), il = (_Float16)(v.y - (float)ih);
-  return {pack_h2(rh, ih), pack_h2(rl, il), pack_h2(ih, -rh), pack_h2(il, -rl)};
+  float m1 = -1.f;
+  asm("" : "+v"(m1));
+  const half2v hv = {(_Float16)v.x, (_Float16)v.y};
+  const half2v lv = {(_Float16)__builtin_fmaf((float)hv.x, m1, v.x), (_Float16)__builtin_fmaf((float)hv.y, m1, v.y)};
+  const half2v hr = {hv.y, -hv.x}, lr = {lv.y, -lv.x};
+  return {__builtin_bit_cast(unsigned, hv), __builtin_bit_cast(unsigned, lv), __builtin_bit_cast(unsigned, hr),
+          __builtin_bit_cast(unsigned, lr)};
 }
 
 // D = C + E_t for window row t of the staged rows (scaled by stage_rows_scaled; row stride
