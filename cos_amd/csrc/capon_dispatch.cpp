@@ -21,8 +21,11 @@ static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
   if constexpr (G == 16 && CPL == 2) {   // covariance work on the matrix cores inside the kernel
     if (TW <= 5 && a.mfma_gram == 1 && !a.R && !a.cov_only)
       return launch_one<float, 16, 2, 5, false, false, 0, false, 1>(a, Mmax, s);
-    if (TW <= 5 && (a.mfma_gram == 0 || a.mfma_gram == 2))
+    if (TW <= 5 && (a.mfma_gram == 0 || a.mfma_gram == 2)) {
+      // rows padded past 128 elements: the instantiation that stages them (zero tail)
+      if (((Mmax + 32 + 4 + 3) / 4) * 4 > 128) return launch_one<float, 16, 2, 5, false, false, 0, false, 3>(a, Mmax, s);
       return launch_one<float, 16, 2, 5, false, false, 0, false, 2>(a, Mmax, s);
+    }
   }
   if (TW <= 5) return launch_one<float, G, CPL, 5, false, false, false>(a, Mmax, s);
   if (TW <= 9) return launch_one<float, G, CPL, 9, false, false, false>(a, Mmax, s);

@@ -113,8 +113,10 @@ template <typename T, int G, int CPL, int TM> struct OccHint {
 // their covariances from per-snapshot Grams on the matrix cores (mfma_gram.h) instead of
 // the displacement recursion; 2 keeps the recursion but forms its rank-two corrections on
 // the matrix cores (FP16 hi/lo-split operands, FP32 accumulation, rows staged with a
-// power-of-two scale) and leaves only the prefix along the diagonals to the VALU.  The
-// launch's other waves keep the VALU recursion.
+// power-of-two scale) and leaves only the prefix along the diagonals to the VALU; 3 is 2
+// for rows padded past 128 elements (M <= 128, padding stored as zeros), kept apart so
+// that the common kernel carries no code for it.  The launch's other waves keep the VALU
+// recursion.
 template <typename T, int G, int CPL, int TM, bool DIRECT, int BWD = 0, bool REF = false, int MG = 0, int WPB = 1>
 __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, const int GREG,
                                             const int gid0, const int npix) {
@@ -196,8 +198,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     const bool consec = __all(mine);
     shared = consec && (NR * MP + NR * LMAX * (int)(sizeof(cplx<T>) / sizeof(float2)) <= NG * GREG);
     if constexpr (MG == 1) mg = consec && L <= 32 && mfma_lds(M - L + 1, NR) <= NG * GREG;
-    if constexpr (MG == 2)
-      mr = shared && L <= 32 && TW <= TM && NR <= 8 && MP <= 128 && NR * MP + NR * LMAX + 1024 <= NG * GREG;
+    if constexpr (MG >= 2)
+      mr = shared && L <= 32 && TW <= TM && NR <= 8 && (MG == 3 ? M <= 128 : MP <= 128) &&
+           NR * MP + NR * LMAX + 1024 <= NG * GREG;
   }
   float2* xs = shared ? xs_all + q * MP : xs_all + q * GREG;
   const float2* const wrow0 = a.x + ((size_t)(p - q) / Nzp * a.Nz + (j - q)) * a.Mstride + off;   // (shared, mg)
@@ -207,8 +210,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     if constexpr (MG)
       pexp = mfma_stage(reinterpret_cast<unsigned*>(xs_all), wrow0, a.Mstride, __builtin_amdgcn_readfirstlane(M), NR,
                         mfma_mpf(__builtin_amdgcn_readfirstlane(M - L + 1)));
-  } else if (MG == 2 && mr) {
-    pexp = stage_rows_scaled<8, 2>(xs_all, MP, wrow0, a.Mstride, NR, M);
+  } else if (MG >= 2 && mr) {
+    pexp = stage_rows_scaled<8, 2, MG == 3>(xs_all, MP, wrow0, a.Mstride, NR, M);
   } else if (shared) {
     stage_rows<8, (WPB > 1 ? 1 : 2)>(xs_all, MP, wrow0, a.Mstride, NR, M, threadIdx.x, 64 * WPB);
   } else {
@@ -406,7 +409,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     constexpr int S = decltype(S_)::value; A[slot<G, S, 0>()] = s0[S]; });
 
     bool rec_done = false;
-    if constexpr (MG == 2) {
+    if constexpr (MG >= 2) {
       if (mr) {
         // the window sums D_q of the rank-two corrections on the matrix cores (mfma_gram.h),
         // pixel q's into rows 1.. of group q's registers through a 32 x 32 buffer, then the
@@ -756,7 +759,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         if (cc < L && R <= cc) {
           cplx<T> v = A[slot<G, S, R>()];
           if (R == cc) v.x += delta;
-          if constexpr (MG == 2 || HSPLIT) v = {ldexpf(v.x, -2 * pexp), ldexpf(v.y, -2 * pexp)};
+          if constexpr (MG >= 2 || HSPLIT) v = {ldexpf(v.x, -2 * pexp), ldexpf(v.y, -2 * pexp)};
           Rp[R * L + cc] = make_float2((float)v.x, (float)-v.y);
           if (R < cc) Rp[cc * L + R] = make_float2((float)v.x, (float)v.y);
         }
@@ -1315,7 +1318,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           zp = {sqrt(pk) / (T)K, T(0)};
         }
       }
-      if constexpr (MG == 2) {   // g came from the scaled rows (w is scale-free)
+      if constexpr (MG >= 2) {   // g came from the scaled rows (w is scale-free)
         if (!steered) zp = {ldexpf(zp.x, -pexp), ldexpf(zp.y, -pexp)};
       }
       if (active && g == 0 && a.z)

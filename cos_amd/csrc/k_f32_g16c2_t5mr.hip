@@ -1,5 +1,5 @@
 // Explicit instantiation of the fused Capon kernel, 16 lanes x 2 columns, windows of up to
-// five rows, the recursion's rank-two corrections on the FP32 matrix cores.
+// five rows, the recursion's rank-two corrections on the matrix cores (FP16 hi/lo split).
 #include "capon_fused_impl.h"
 
 namespace cosk {

@@ -159,13 +159,14 @@ __device__ __forceinline__ void gram_pass(const unsigned* F, int t, int MPF, int
 // measured slower: 1.617 against 1.536 ms per frame, the operand selection for four inner
 // indices costing more VALU than the shorter chains saved.)
 
-// Stages the wave's nrows <= RB rows of M elements (src + t sstride, zero padded to MP
-// <= 64 CB) scaled by 2^e, e chosen so that the largest |re| or |im| lies in [2^7, 2^8)
+// Stages the wave's nrows <= RB rows of M <= 64 CB elements (src + t sstride, zero padded
+// to MP) scaled by 2^e, e chosen so that the largest |re| or |im| lies in [2^7, 2^8)
 // (e = 0 for all-zero or non-finite data), and returns e: a power-of-two scale is exact,
 // so everything built from the rows is 2^e (or 2^(2e) for second moments) times the
 // unscaled result, and the FP16 split below keeps its 22 significant bits for every
-// value down to 2^-14 of the largest.  All loads are issued before the reduction.
-template <int RB, int CB>
+// value down to 2^-14 of the largest.  All loads are issued before the reduction; with
+// ZTAIL, padding past 64 CB is stored as zeros without loads (otherwise MP <= 64 CB).
+template <int RB, int CB, bool ZTAIL = false>
 __device__ __forceinline__ int stage_rows_scaled(float2* dst, int MP, const float2* src, size_t sstride, int nrows,
                                                  int M) {
   const int l = threadIdx.x & 63;
@@ -191,6 +192,9 @@ __device__ __forceinline__ int stage_rows_scaled(float2* dst, int MP, const floa
       const int m = l + 64 * c;
       if (r < nrows && m < MP) dst[r * MP + m] = make_float2(v[r][c].x * sc, v[r][c].y * sc);
     }
+  if constexpr (ZTAIL)
+    for (int m = 64 * CB + l; m < MP; m += 64)
+      for (int r = 0; r < nrows; ++r) dst[r * MP + m] = make_float2(0.f, 0.f);
   return e;
 }
 

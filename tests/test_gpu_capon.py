@@ -193,14 +193,15 @@ def test_cfg2_slice_refined_power(cfg2_slice):
 
 @pytest.mark.parametrize("mode", ["fused", "rank", "valu"])
 @pytest.mark.parametrize("M,L,Y,Nx,Nz", [(64, 32, 2, 2, 40), (48, 24, 1, 3, 37), (40, 20, 0, 2, 30), (80, 32, 2, 2, 44),
-                                       (33, 17, 2, 1, 21), (64, 32, 2, 3, 43)])
+                                       (33, 17, 2, 1, 21), (64, 32, 2, 3, 43), (120, 32, 2, 2, 40), (96, 24, 1, 2, 34)])
 def test_fused_mfma_covariance_matches_oracle(M, L, Y, Nx, Nz, mode):
     """Covariance work on the matrix cores inside the fused kernel ("fused": per-snapshot
     Grams from an FP16 hi/lo split of power-of-two scaled rows; "rank": the recursion's
-    rank-two corrections summed per window in FP32; "valu": none, the recursion on the
-    VALU): P, w and z against the FP64 oracle, on shapes whose waves take the matrix-core
-    path (four consecutive rows) and, where the row count is not a multiple of four, the
-    recursion at the beam ends."""
+    rank-two corrections per window from FP16 hi/lo-split products of scaled rows, rows
+    padded past 128 elements (M = 120, 96) through the kernel that stages them; "valu":
+    none, the recursion on the VALU): P, w and z against the FP64 oracle, on shapes whose
+    waves take the matrix-core path (four consecutive rows) and, where the row count is not
+    a multiple of four, the recursion at the beam ends."""
     say("fused mfma covariance", mode, M, L, Y, Nx, Nz)
     x = uniform_cube(M * 7 + L + Y, Nx, Nz, M)
     o, _ = _plan_run(x, 0.05, L, Y, cov_mfma=mode)
@@ -222,6 +223,28 @@ def test_fused_mfma_covariance_headline_slice(cfg2_slice, mode):
     e = rel(o["P"], ref["P"].ravel())
     print(f"P rel err max {e.max():.3e}", flush=True)
     assert e.max() <= 1e-5
+
+
+@pytest.mark.parametrize("e", [-40, 50])
+@pytest.mark.parametrize("M,L,Y", [(64, 32, 2), (120, 32, 2), (128, 64, 2)])
+def test_split_recursion_is_scale_free(M, L, Y, e):
+    """The FP16 hi/lo split of the rank-two products works on rows normalised by a power of
+    two (the largest component into [2^7, 2^8)), so data multiplied by 2^e must give P times
+    2^(2e), z times 2^e and the same w, bit for bit: the split sees the same operands and
+    every FP32 step on the scaled data is exact.  Shapes: the 16-lane kernel (M = 64), its
+    wide-row instantiation (M = 120), the 64-lane kernel (L = 64, its scale folded into the
+    first row and the trace); Nz - 2Y a multiple of four, so every wave takes the path.
+    (2^-40 and 2^50 keep P a normal float: its scale-back is then exact.)"""
+    say("split recursion scale", M, L, Y, e)
+    x = uniform_cube(M + 5 * L + Y, 2, 32 + 2 * Y, M)
+    xs = (x * np.float32(2.0 ** e)).astype(np.complex64)
+    o0, _ = _plan_run(x, 0.05, L, Y)
+    o1, _ = _plan_run(xs, 0.05, L, Y)
+    assert np.array_equal(o1["P"], (o0["P"].astype(np.float64) * 2.0 ** (2 * e)).astype(np.float32))
+    assert np.array_equal(o1["z"], (o0["z"].astype(np.complex128) * 2.0 ** e).astype(np.complex64))
+    assert np.array_equal(o1["w"], o0["w"])
+    ref = O.capon(x.astype(np.complex128), 0.05, L, Y)
+    assert rel(o0["P"], ref["P"]).max() <= 1e-5
 
 
 def test_rank_corrections_covariance_output():
