@@ -974,16 +974,51 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         // diagonal, which nothing reads.
         const bool lo = (threadIdx.x & 32) == 0;
         f32x16 cr = {}, ci = {};
-        static_for<0, 32>([&](auto M_) {
-          constexpr int m = decltype(M_)::value;
-          const unsigned ur = __float_as_uint((float)A[m].x), ui = __float_as_uint((float)A[m].y);
-          const float urd = __uint_as_float(__builtin_amdgcn_permlane32_swap(ur, ur, false, false)[1]);
-          const float uid = __uint_as_float(__builtin_amdgcn_permlane32_swap(ui, ui, false, false)[1]);
-          const float X = lo ? urd : (float)A[m].y;
-          const float Y = lo ? -uid : (float)A[m].x;
-          cr = __builtin_amdgcn_mfma_f32_32x32x2f32(X, X, cr, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_32x32x2f32(Y, X, ci, 0, 0, 0);
-        });
+        float sre = 1.f, sim = 1.f;   // what the sums are multiplied by when applied
+        if constexpr (HSPLIT) {
+          // FP16 hi/lo-split operands, two steps per v_mfma_f32_32x32x16_f16 (32 cycles)
+          // instead of one per v_mfma_f32_32x32x2_f32 (64): inner index k = 8 (l / 32) + s,
+          //   A (rows)    = [h_m l_m h_m' l_m'] in both halves,
+          //   B (columns) = [h_m l_m h_m' l_m'] (l < 32),  [l_m h_m l_m' h_m'] (l >= 32),
+          // so one half takes hi.hi + lo.lo and the other hi.lo + lo.hi of each (re, im) pair
+          // (m' = m + 1); the imaginary part's A holds the pairs rotated to (im, -re), which
+          // gives -Im.  Every entry of U satisfies |U[m][c]|^2 <= S_L[c][c] <= tr S_L, so the
+          // operands are scaled by 2^eu with 2^eu sqrt(tr S_L) below 2^14 (FP16 holds them
+          // with room; the trace is the same in every lane); the sums are 2^(2 eu) times the
+          // true ones.  Lanes of columns 32.. hold the operands; one half-wave swap hands
+          // lane l < 32 those of column 32 + l.
+          int et = 0;
+          (void)frexpf((float)(trS + (T)L * delta), &et);   // tr S_L < 2^et
+          const int eu = (trS > T(0) && trS < T(3.0e38)) ? 14 - ((et + 1) >> 1) : 0;
+          const float su = ldexpf(1.f, eu);
+          sre = -ldexpf(1.f, -2 * eu);
+          sim = -sre;
+          auto up = [](unsigned x, unsigned y) {   // lanes l < 32 get x of lane 32 + l, the rest their own y
+            return __builtin_amdgcn_permlane32_swap(x, y, false, false)[1];
+          };
+          static_for<0, 16>([&](auto P_) {
+            constexpr int m = 2 * decltype(P_)::value;
+            const SplitC s0 = split_c(make_float2((float)A[m].x * su, (float)A[m].y * su));
+            const SplitC s1 = split_c(make_float2((float)A[m + 1].x * su, (float)A[m + 1].y * su));
+            const u32x4 Am = {up(s0.h, s0.h), up(s0.l, s0.l), up(s1.h, s1.h), up(s1.l, s1.l)};
+            const u32x4 Im = {up(s0.hr, s0.hr), up(s0.lr, s0.lr), up(s1.hr, s1.hr), up(s1.lr, s1.lr)};
+            const u32x4 Bm = {up(s0.h, s0.l), up(s0.l, s0.h), up(s1.h, s1.l), up(s1.l, s1.h)};
+            cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(Am), as_h8(Bm), cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_h8(Im), as_h8(Bm), ci, 0, 0, 0);
+          });
+        } else {
+          sim = sre = -1.f;
+          static_for<0, 32>([&](auto M_) {
+            constexpr int m = decltype(M_)::value;
+            const unsigned ur = __float_as_uint((float)A[m].x), ui = __float_as_uint((float)A[m].y);
+            const float urd = __uint_as_float(__builtin_amdgcn_permlane32_swap(ur, ur, false, false)[1]);
+            const float uid = __uint_as_float(__builtin_amdgcn_permlane32_swap(ui, ui, false, false)[1]);
+            const float X = lo ? urd : (float)A[m].y;
+            const float Y = lo ? -uid : (float)A[m].x;
+            cr = __builtin_amdgcn_mfma_f32_32x32x2f32(X, X, cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x2f32(Y, X, ci, 0, 0, 0);
+          });
+        }
         // accumulator element v of lane l: column l % 32, row 8(v/4) + 4(l/32) + v%4 of the
         // block; the lower half's rows move up to the lanes of their columns
         static_for<0, 16>([&](auto V_) {
@@ -992,10 +1027,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           const unsigned br = __float_as_uint(cr[v]), bi = __float_as_uint(ci[v]);
           const float lr = __uint_as_float(__builtin_amdgcn_permlane32_swap(br, br, false, false)[0]);
           const float li = __uint_as_float(__builtin_amdgcn_permlane32_swap(bi, bi, false, false)[0]);
-          A[rhi].x -= (T)cr[v];
-          A[rhi].y -= (T)ci[v];
-          A[rlo].x -= (T)lr;
-          A[rlo].y -= (T)li;
+          A[rhi].x = fma((T)cr[v], (T)sre, A[rhi].x);
+          A[rhi].y = fma((T)ci[v], (T)sim, A[rhi].y);
+          A[rlo].x = fma((T)lr, (T)sre, A[rlo].x);
+          A[rlo].y = fma((T)li, (T)sim, A[rlo].y);
         });
       }
     }
