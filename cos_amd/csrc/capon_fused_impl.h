@@ -1284,6 +1284,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         cplx<T> r[CPL];
         static_for<0, CPL>([&](auto S_) {
           constexpr int S = decltype(S_)::value;
+          // matrix-core waves solved with S (and delta) 2^(2 pexp) times the data's, so y is
+          // 2^(-2 pexp) times the true solution: the residual a - S_true y_true - delta_true
+          // y_true takes S y from the unscaled rows times 2^(2 pexp); dy then comes out of the
+          // same scaled factor, and y + dy keeps the scale
+          if constexpr (MG >= 2) sy[S] = {ldexpf(sy[S].x, 2 * pexp), ldexpf(sy[S].y, 2 * pexp)};
           r[S] = cs[S] < L ? cplx<T>{av[S].x - sy[S].x - delta * y[S].x, av[S].y - sy[S].y - delta * y[S].y}
                            : cplx<T>{T(0), T(0)};
         });
@@ -1302,8 +1307,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           if (cs[S] < L) ay += av[S].x * y[S].x + av[S].y * y[S].y;   // Re(a^H y)
         });
         accP = Gp::sum(ay);   // Re(a^H y) of the refined solution
-        if (active && g == 0 && a.P && (!OPTS || a.pmode == 0))
-          a.P[po] = valid ? (float)(T(1) / (norm * accP)) : nanv;
+        if (active && g == 0 && a.P && (!OPTS || a.pmode == 0)) {
+          float pv = (float)(T(1) / (norm * accP));
+          if constexpr (MG != 0) pv = ldexpf(pv, -2 * pexp);
+          a.P[po] = valid ? pv : nanv;
+        }
       }
       // w = y / (a^H y), a^H y = |v|^2 (refined: its real part);  z = w^H g
       const T sc = T(1) / accP;

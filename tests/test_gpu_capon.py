@@ -247,6 +247,24 @@ def test_split_recursion_is_scale_free(M, L, Y, e):
     assert rel(o0["P"], ref["P"]).max() <= 1e-5
 
 
+@pytest.mark.parametrize("e", [-40, 50])
+def test_split_recursion_refined_is_scale_free(e):
+    """The refinement kernel on the split recursion forms its residual from the unscaled
+    rows and the scaled solve: data times 2^e must still give P times 2^(2e), z times 2^e
+    and the same w, bit for bit, and the refined power must beat the unrefined one."""
+    say("split recursion refined scale", 64, 32, 2, e)
+    x = uniform_cube(417, 2, 36, 64)
+    xs = (x * np.float32(2.0 ** e)).astype(np.complex64)
+    o0, _ = _plan_run(x, 0.05, 32, 2, refine=True)
+    o1, _ = _plan_run(xs, 0.05, 32, 2, refine=True)
+    assert np.array_equal(o1["P"], (o0["P"].astype(np.float64) * 2.0 ** (2 * e)).astype(np.float32))
+    assert np.array_equal(o1["z"], (o0["z"].astype(np.complex128) * 2.0 ** e).astype(np.complex64))
+    assert np.array_equal(o1["w"], o0["w"])
+    ref = O.capon(x.astype(np.complex128), 0.05, 32, 2)
+    u, _ = _plan_run(x, 0.05, 32, 2)
+    assert rel(o0["P"], ref["P"]).max() <= max(rel(u["P"], ref["P"]).max(), 1e-7)
+
+
 def test_rank_corrections_covariance_output():
     """The rank-two corrections on the matrix cores with the materialised R (the
     reference layout) against the oracle's."""
