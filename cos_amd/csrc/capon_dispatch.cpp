@@ -107,8 +107,11 @@ hipError_t launch_capon_bwd(const CaponLaunch& a, int Lmax, int Mmax, hipStream_
     return w5 ? launch_one<float, 16, 1, 5, false, false, 128>(a, Mmax, s) : launch_one<float, 16, 1, 0, false, false, 128>(a, Mmax, s);
   }
   if (Lmax <= 32) {
-    if (Mmax <= 64)
+    if (Mmax <= 64) {
+      // the forward recomputation's rank-two terms on the matrix cores
+      if (w5 && (a.mfma_gram == 0 || a.mfma_gram == 2)) return launch_one<float, 16, 2, 5, false, false, 64, false, 2>(a, Mmax, s);
       return w5 ? launch_one<float, 16, 2, 5, false, false, 64>(a, Mmax, s) : launch_one<float, 16, 2, 0, false, false, 64>(a, Mmax, s);
+    }
     return w5 ? launch_one<float, 16, 2, 5, false, false, 128>(a, Mmax, s) : launch_one<float, 16, 2, 0, false, false, 128>(a, Mmax, s);
   }
   if (why) *why = "the adjoint supports L <= 32";

@@ -1383,7 +1383,12 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         const T gPp = (active && valid) ? (T)a.gP[p] : T(0);
         const T coeff = -gPp / (norm * accP * accP);            // dl/dq
         const T cload = (T)dload * yn / (T)L;
-        if (active && g == 0 && a.gd) a.gd[p] = (float)(coeff * (-trS * yn / (T)L));
+        // (matrix-core waves: y, accP and trS carry 2^(-2 pexp), 2^(-2 pexp) and 2^(2 pexp),
+        // so coeff and the products below with y and the unscaled rows cancel theirs, and
+        // only trS is taken back)
+        T trS_d = trS;
+        if constexpr (MG >= 2) trS_d = ldexpf(trS, -2 * pexp);
+        if (active && g == 0 && a.gd) a.gd[p] = (float)(coeff * (-trS_d * yn / (T)L));
         // the window rows again (the group's region held U), then beta after them with
         // LMAX zeros in front and zeros for K <= k < KS*G, so the convolution below
         // needs no bounds tests
@@ -1535,7 +1540,7 @@ template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD
           int MG = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(64 * Wpb<T, G, TM, PERSIST, BWD, REF>::value,
                                                       64 * Wpb<T, G, TM, PERSIST, BWD, REF>::value),
-                          amdgpu_waves_per_eu(BWD ? 1
+                          amdgpu_waves_per_eu(BWD ? (MG ? 2 : 1)
                                               : ((REF && G == 64) || Wpb<T, G, TM, PERSIST, BWD, REF>::value > 1)
                                                   ? 2
                                                   : OccHint<T, G, CPL, TM>::value)))
