@@ -817,7 +817,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // 16 x 16 x 1 does the work of 16 FMA instructions: a win for this shape, which runs one
   // wave per SIMD and so issues at most one VALU instruction per four cycles, and neutral
   // for two columns per lane (two waves per SIMD interleave their VALU work at about 2.6
-  // cycles per instruction; 1.40 ms per headline frame either way), which keeps the VALU.
+  // cycles per instruction; the headline frame took the same time either way), which keeps
+  // the VALU.
   constexpr bool MBLK = DPP_FMA && CPL >= 3;
   constexpr int NBK = MBLK ? CPL * (CPL - 1) / 2 : 1;
   f32x16 bkr[NBK], bki[NBK];
