@@ -32,6 +32,10 @@ static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
       return launch_one<float, 16, 2, 5, false, false, 0, false, 2>(a, Mmax, s);
     }
   }
+  if constexpr (G == 16 && CPL == 3) {   // rows and columns 0..31 of the recursion on the matrix cores
+    if (TW <= 5 && (a.mfma_gram == 0 || a.mfma_gram == 2))
+      return launch_one<float, 16, 3, 5, false, false, 0, false, 2>(a, Mmax, s);
+  }
   if (TW <= 5) return launch_one<float, G, CPL, 5, false, false, false>(a, Mmax, s);
   if (TW <= 9) return launch_one<float, G, CPL, 9, false, false, false>(a, Mmax, s);
   return launch_one<float, G, CPL, 0, false, false, false>(a, Mmax, s);

@@ -130,6 +130,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // one pixel per wave, FP32, fixed window bound, no refinement: the recursion's rank-two
   // sums on FP16 hi/lo-split matrix-core products (the refinement kernels keep FP32 ones)
   constexpr bool HSPLIT = G == 64 && CPL == 1 && TM > 0 && std::is_same<T, float>::value && !REF && !BWD;
+  // matrix-core recursion with rows padded past 128 elements: staged with a zero tail
+  constexpr bool ZT = MG == 3 || CPL == 3;
   extern __shared__ float2 xs_all[];
 
   const int g = Gp::lane(), q = WPB > 1 ? (int)(threadIdx.x >> 6) : Gp::group();
@@ -198,8 +200,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     const bool consec = __all(mine);
     shared = consec && (NR * MP + NR * LMAX * (int)(sizeof(cplx<T>) / sizeof(float2)) <= NG * GREG);
     if constexpr (MG == 1) mg = consec && L <= 32 && mfma_lds(M - L + 1, NR) <= NG * GREG;
+    // (three columns per lane: the matrix cores take the block of rows and columns 0..31
+    // for any L, the VALU the rest)
     if constexpr (MG >= 2)
-      mr = shared && L <= 32 && TW <= TM && NR <= 8 && (MG == 3 ? M <= 128 : MP <= 128) &&
+      mr = shared && (CPL == 3 || L <= 32) && TW <= TM && NR <= 8 && (ZT ? M <= 128 : MP <= 128) &&
            NR * MP + NR * LMAX + 1024 <= NG * GREG;
   }
   float2* xs = shared ? xs_all + q * MP : xs_all + q * GREG;
@@ -211,7 +215,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       pexp = mfma_stage(reinterpret_cast<unsigned*>(xs_all), wrow0, a.Mstride, __builtin_amdgcn_readfirstlane(M), NR,
                         mfma_mpf(__builtin_amdgcn_readfirstlane(M - L + 1)));
   } else if (MG >= 2 && mr) {
-    pexp = stage_rows_scaled<8, 2, MG == 3>(xs_all, MP, wrow0, a.Mstride, NR, M);
+    pexp = stage_rows_scaled<8, 2, ZT>(xs_all, MP, wrow0, a.Mstride, NR, M);
   } else if (shared) {
     stage_rows<8, (WPB > 1 ? 1 : 2)>(xs_all, MP, wrow0, a.Mstride, NR, M, threadIdx.x, 64 * WPB);
   } else {
@@ -233,7 +237,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   auto window_take = [&](auto R0_, const float2* Tb) {
     constexpr int R0 = decltype(R0_)::value;
     if constexpr (MG != 0) {
-      static_assert(G == 16 && CPL == 2, "the matrix-core covariance paths are for 16 lanes x 2 columns");
+      static_assert(G == 16 && CPL >= 2, "the matrix-core covariance paths are for 16 lanes x 2 or 3 columns");
       static_for<R0, 32>([&](auto R_) {
         constexpr int R = decltype(R_)::value;
         if constexpr (R < 16) {
@@ -408,7 +412,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     static_for<0, CPL>([&](auto S_) {
     constexpr int S = decltype(S_)::value; A[slot<G, S, 0>()] = s0[S]; });
 
-    bool rec_done = false;
+    bool rec_done = false, hi_only = false;
     if constexpr (MG >= 2) {
       if (mr) {
         // the window sums D_q of the rank-two corrections on the matrix cores (mfma_gram.h),
@@ -449,9 +453,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
             take(sr, si, qq);
           }
         }
-        static_for<1, LMAX>([&](auto R_) {
+        static_for<1, 32>([&](auto R_) {
           constexpr int R = decltype(R_)::value;
-          static_for<0, CPL>([&](auto S_) {
+          static_for<0, 2>([&](auto S_) {
             constexpr int S = decltype(S_)::value;
             if constexpr (R < G * (S + 1)) {
               cplx<T> own = {Gp::ror1(A[slot<G, S, R - 1>()].x), Gp::ror1(A[slot<G, S, R - 1>()].y)};
@@ -465,7 +469,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
             }
           });
         });
-        rec_done = true;
+        // three columns per lane: the third slot (columns 32..47) by the VALU recursion below
+        if constexpr (CPL == 2) rec_done = true;
+        else hi_only = true;
       }
     }
     if (!rec_done) {
@@ -590,12 +596,16 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         A[R].x += Gp::ror1(A[R - 1].x);
         A[R].y += Gp::ror1(A[R - 1].y);
       });
-    } else
+    } else {
+    // the recursion on the VALU for the slots from S0 on (S0 = 2 after the matrix cores
+    // took the first two)
+    auto valu_rec = [&](auto S0_) {
+    constexpr int S0 = decltype(S0_)::value;
     static_for<1, LMAX>([&](auto R_) {
     constexpr int R = decltype(R_)::value;
       if (SKIP && R >= Lw) return;
       cplx<T> nv[CPL];
-      static_for<0, CPL>([&](auto S_) {
+      static_for<S0, CPL>([&](auto S_) {
     constexpr int S = decltype(S_)::value;
         if constexpr (R < G * (S + 1)) {
           cplx<T> own = {Gp::ror1(A[slot<G, S, R - 1>()].x), Gp::ror1(A[slot<G, S, R - 1>()].y)};
@@ -619,7 +629,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           u1 = ldx<T>(xs + t * MP + K - 1 + R);
           u0 = ldx<T>(xs + t * MP + R - 1);
         }
-        static_for<0, CPL>([&](auto S_) {
+        static_for<S0, CPL>([&](auto S_) {
     constexpr int S = decltype(S_)::value;
           if constexpr (R < G * (S + 1)) {
             add_mul_conj(nv[S], u1, alt[S]);
@@ -645,11 +655,19 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           corr(t, alt, bet);
         }
       }
-      static_for<0, CPL>([&](auto S_) {
+      static_for<S0, CPL>([&](auto S_) {
     constexpr int S = decltype(S_)::value;
         if constexpr (R < G * (S + 1)) A[slot<G, S, R>()] = nv[S];
       });
     });
+    };
+    if constexpr (CPL == 3 && MG >= 2) {
+      if (hi_only) valu_rec(ic<2>{});
+      else valu_rec(ic<0>{});
+    } else {
+      valu_rec(ic<0>{});
+    }
+    }
     }
   }
 
