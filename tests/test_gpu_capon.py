@@ -226,14 +226,16 @@ def test_fused_mfma_covariance_headline_slice(cfg2_slice, mode):
 
 
 @pytest.mark.parametrize("e", [-40, 50])
-@pytest.mark.parametrize("M,L,Y", [(64, 32, 2), (120, 32, 2), (128, 64, 2)])
+@pytest.mark.parametrize("M,L,Y", [(64, 32, 2), (120, 32, 2), (128, 64, 2), (96, 40, 2)])
 def test_split_recursion_is_scale_free(M, L, Y, e):
     """The FP16 hi/lo split of the rank-two products works on rows normalised by a power of
     two (the largest component into [2^7, 2^8)), so data multiplied by 2^e must give P times
     2^(2e), z times 2^e and the same w, bit for bit: the split sees the same operands and
     every FP32 step on the scaled data is exact.  Shapes: the 16-lane kernel (M = 64), its
     wide-row instantiation (M = 120), the 64-lane kernel (L = 64, its scale folded into the
-    first row and the trace); Nz - 2Y a multiple of four, so every wave takes the path.
+    first row and the trace), three columns per lane (L = 40: rows and columns 0..31 on the
+    matrix cores, the third slot on the VALU); Nz - 2Y a multiple of four, so every wave
+    takes the path.
     (2^-40 and 2^50 keep P a normal float: its scale-back is then exact.)"""
     say("split recursion scale", M, L, Y, e)
     x = uniform_cube(M + 5 * L + Y, 2, 32 + 2 * Y, M)
