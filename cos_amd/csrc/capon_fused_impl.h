@@ -127,9 +127,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // fixed-window kernels keep their registers
   constexpr bool OPTS = TM == 0 && !BWD;
   constexpr int TMR = TM > 0 ? TM : 1;
-  // one pixel per wave, FP32, fixed window bound, no refinement: the recursion's rank-two
-  // sums on FP16 hi/lo-split matrix-core products (the refinement kernels keep FP32 ones)
-  constexpr bool HSPLIT = G == 64 && CPL == 1 && TM > 0 && std::is_same<T, float>::value && !REF && !BWD;
+  // one pixel per wave, FP32, fixed window bound: the recursion's rank-two sums on FP16
+  // hi/lo-split matrix-core products
+  constexpr bool HSPLIT = G == 64 && CPL == 1 && TM > 0 && std::is_same<T, float>::value && !BWD;
   // matrix-core recursion with rows padded past 128 elements: staged with a zero tail
   constexpr bool ZT = MG == 3 || CPL == 3;
   extern __shared__ float2 xs_all[];
@@ -1307,7 +1307,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           // 2^(-2 pexp) times the true solution: the residual a - S_true y_true - delta_true
           // y_true takes S y from the unscaled rows times 2^(2 pexp); dy then comes out of the
           // same scaled factor, and y + dy keeps the scale
-          if constexpr (MG >= 2) sy[S] = {ldexpf(sy[S].x, 2 * pexp), ldexpf(sy[S].y, 2 * pexp)};
+          if constexpr (MG >= 2 || HSPLIT) sy[S] = {ldexpf(sy[S].x, 2 * pexp), ldexpf(sy[S].y, 2 * pexp)};
           r[S] = cs[S] < L ? cplx<T>{av[S].x - sy[S].x - delta * y[S].x, av[S].y - sy[S].y - delta * y[S].y}
                            : cplx<T>{T(0), T(0)};
         });
@@ -1328,7 +1328,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         accP = Gp::sum(ay);   // Re(a^H y) of the refined solution
         if (active && g == 0 && a.P && (!OPTS || a.pmode == 0)) {
           float pv = (float)(T(1) / (norm * accP));
-          if constexpr (MG != 0) pv = ldexpf(pv, -2 * pexp);
+          if constexpr (MG != 0 || HSPLIT) pv = ldexpf(pv, -2 * pexp);
           a.P[po] = valid ? pv : nanv;
         }
       }
