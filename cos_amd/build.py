@@ -24,7 +24,7 @@ SOURCES = [
     "k_f32_g64c1_t5.hip", "k_f32_g64c1_t9.hip", "k_f32_g64c1_t0.hip",
     "k_f32_g16c2_t5.hip", "k_f32_g16c2_t5mg.hip", "k_f32_g16c2_t5mr.hip", "k_f32_g16c2_t5mw.hip",
     "k_f32_g16c2_t9.hip", "k_f32_g16c2_t0.hip",
-    "k_f32_g16c1.hip", "k_f32_g16c3.hip", "k_f32_g16c3_t5mr.hip", "k_f32_ref_c1.hip", "k_f32_ref_c2.hip", "k_f32_ref_c3.hip", "k_f32_ref_g64.hip", "k_f32_ref5_c1.hip", "k_f32_ref5_c2.hip", "k_f32_ref5_c2mr.hip", "k_f32_ref5_c3.hip", "k_f32_ref5_g64.hip", "k_f64_c1.hip", "k_f64_c2.hip", "k_f64r_c1.hip", "k_f64r_c2.hip", "k_f64_g64.hip", "k_f64_g64_t5.hip", "k_bwd_c1.hip", "k_bwd_c2_m64.hip", "k_bwd_c2_m64mr.hip", "k_bwd_c2_m128.hip", "components.hip", "wide.hip", "cov_mfma.hip", "beamspace.hip", "tdib.hip", "lca.hip", "capon_dispatch.cpp", "capon_api.cpp",
+    "k_f32_g16c1.hip", "k_f32_g16c3.hip", "k_f32_g16c3_t5mr.hip", "k_f32_ref_c1.hip", "k_f32_ref_c2.hip", "k_f32_ref_c3.hip", "k_f32_ref_g64.hip", "k_f32_ref5_c1.hip", "k_f32_ref5_c2.hip", "k_f32_ref5_c2mr.hip", "k_f32_ref5_c2mw.hip", "k_f32_ref5_c3.hip", "k_f32_ref5_c3mr.hip", "k_f32_ref5_g64.hip", "k_f64_c1.hip", "k_f64_c2.hip", "k_f64r_c1.hip", "k_f64r_c2.hip", "k_f64_g64.hip", "k_f64_g64_t5.hip", "k_bwd_c1.hip", "k_bwd_c2_m64.hip", "k_bwd_c2_m64mr.hip", "k_bwd_c2_m128.hip", "components.hip", "wide.hip", "cov_mfma.hip", "beamspace.hip", "tdib.hip", "lca.hip", "capon_dispatch.cpp", "capon_api.cpp",
 ]
 
 COMMON = [

@@ -14,8 +14,13 @@ static hipError_t launch_f32_tm(const CaponLaunch& a, int Mmax, hipStream_t s) {
   if (a.refine) {
     if (TW <= 5 && !(a.fb || a.zmode || a.pmode || a.Rin)) {
       if constexpr (G == 16 && CPL == 2) {   // the recursion's rank-two terms on the matrix cores
-        if ((a.mfma_gram == 0 || a.mfma_gram == 2) && ((Mmax + 32 + 4 + 3) / 4) * 4 <= 128)
-          return launch_one<float, 16, 2, 5, false, false, 0, true, 2>(a, Mmax, s);
+        if (a.mfma_gram == 0 || a.mfma_gram == 2) {
+          if (((Mmax + 32 + 4 + 3) / 4) * 4 <= 128) return launch_one<float, 16, 2, 5, false, false, 0, true, 2>(a, Mmax, s);
+          return launch_one<float, 16, 2, 5, false, false, 0, true, 3>(a, Mmax, s);   // rows padded past 128
+        }
+      }
+      if constexpr (G == 16 && CPL == 3) {   // rows and columns 0..31 on the matrix cores
+        if (a.mfma_gram == 0 || a.mfma_gram == 2) return launch_one<float, 16, 3, 5, false, false, 0, true, 2>(a, Mmax, s);
       }
       return launch_one<float, G, CPL, 5, false, false, 0, true>(a, Mmax, s);
     }

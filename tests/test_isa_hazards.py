@@ -19,8 +19,9 @@ HIPCC = B.HIPCC
 # every FP32 16-lane kernel (the ones with DPP-broadcast FMAs)
 SOURCES = ["k_f32_g16c2_t5mr.hip", "k_f32_g16c2_t5mw.hip", "k_f32_g16c2_t5.hip", "k_f32_g16c2_t9.hip", "k_f32_g16c2_t0.hip",
            "k_f32_g16c2_t5mg.hip", "k_f32_g16c1.hip", "k_f32_g16c3.hip", "k_f32_g16c3_t5mr.hip", "k_f32_ref_c1.hip", "k_f32_ref_c2.hip",
-           "k_f32_ref_c3.hip", "k_f32_ref5_c1.hip", "k_f32_ref5_c2.hip", "k_f32_ref5_c2mr.hip", "k_f32_ref5_c3.hip",
-           "k_bwd_c1.hip", "k_bwd_c2_m64.hip", "k_bwd_c2_m64mr.hip", "k_bwd_c2_m128.hip"]
+           "k_f32_ref_c3.hip", "k_f32_ref5_c1.hip", "k_f32_ref5_c2.hip", "k_f32_ref5_c2mr.hip", "k_f32_ref5_c2mw.hip",
+           "k_f32_ref5_c3.hip", "k_f32_ref5_c3mr.hip", "k_bwd_c1.hip", "k_bwd_c2_m64.hip", "k_bwd_c2_m64mr.hip",
+           "k_bwd_c2_m128.hip"]
 
 DPP = re.compile(r"^(v_\w+_dpp)\s+(v\d+),\s*-?\|?(v\d+)")
 DST = re.compile(r"^v_\w+\s+(v\d+|v\[\d+:\d+\])")
