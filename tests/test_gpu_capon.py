@@ -415,7 +415,8 @@ def test_mfma_covariance_aperture_rows():
 SHAPE_CAP = (16, 32, 64, 48)   # lane shapes 1..4: the largest L each holds
 
 
-@pytest.mark.parametrize("M,L,Y", [(24, 12, 2), (64, 32, 2), (96, 40, 2), (96, 48, 2), (80, 36, 4), (56, 33, 0)])
+@pytest.mark.parametrize("M,L,Y", [(24, 12, 2), (64, 32, 2), (96, 40, 2), (96, 48, 2), (80, 36, 4), (56, 33, 0),
+                                   (48, 48, 1), (40, 34, 2), (32, 32, 2)])
 def test_every_lane_shape_matches_oracle(M, L, Y):
     """Each lane shape that holds L, forced through the tuning table, against the FP64
     oracle (windows of 1, 5 and 9 rows: the three fixed-window kernels); the table is
