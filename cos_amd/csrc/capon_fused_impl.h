@@ -812,6 +812,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // one pixel per wave in FP32 with 64 rows: the trailing block of rows and columns 32..63
   // is updated by the first 32 steps at once on the matrix cores
   constexpr bool BLK = std::is_same<T, float>::value && G == 64 && CPL == 1;
+  constexpr bool BLK2 = BLK && HSPLIT && !REF;
   constexpr int CRR = CholRegRows<G>::value;
   constexpr bool PUBLISH = !DPP_FMA && CRR < LMAX - 1;
   cplx<T>* const Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
@@ -848,6 +849,62 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           constexpr int s = decltype(S_)::value;
           constexpr int b = (ib - 1) * CPL - (ib - 1) * ib / 2 + (s - ib);
           f(ic<b>{}, ic<ib>{}, ic<s>{});
+        });
+      });
+    }
+  };
+  // BLK2: rows 16 rb .. 16 rb + 15 of every column 16 rb.. minus the 16 rank-one terms of
+  // steps 16 (rb - 1) .. 16 rb - 1, on v_mfma_f32_16x16x32_f16 (one 16 x 16 column tile jb
+  // per instruction; inner index k = 8 (l / 16) + s, so each 16-lane chunk g of the wave
+  // carries one step).  For four steps 4t + g at a time, a 4 x 4 transpose of 16-lane
+  // chunks (xpose4) of the step rows leaves, in register jb, column tile jb's values of
+  // step 4t + g in chunk g: the layout of both operands, rows (tile rb) and columns (tile
+  // jb).  Each (re, im) pair is split hi + lo in FP16 with
+  //   A (rows) = [h l h l],  B (columns) = [h l l h]
+  // (hi.hi + lo.lo + hi.lo + lo.hi); the imaginary part's A holds the pairs rotated to
+  // (im, -re), which gives -Im.  The operands are scaled by 2^eu with 2^eu sqrt(tr S_L)
+  // below 2^14 (|U[m][c]|^2 <= tr S_L) and the sums applied times 2^(-2 eu).  The results
+  // go back by the same transpose: register g of set r then holds, in chunk jb, row
+  // 16 rb + 4g + r of column 16 jb + l % 16 (chunks below rb receive zeros).
+  auto sub_update = [&](auto RB_) {
+    if constexpr (BLK2) {
+      constexpr int rb = decltype(RB_)::value, m0 = 16 * (rb - 1);
+      int et = 0;
+      (void)frexpf((float)(trS + (T)L * delta), &et);   // tr S_L < 2^et
+      const int eu = (trS > T(0) && trS < T(3.0e38)) ? 14 - ((et + 1) >> 1) : 0;
+      const float su = ldexpf(1.f, eu), sre = -ldexpf(1.f, -2 * eu), sim = -sre;
+      f32x4v dr[4] = {}, di[4] = {};
+      static_for<0, 4>([&](auto T_) {
+        constexpr int t = decltype(T_)::value;
+        float xr[4], xi[4];
+        static_for<0, 4>([&](auto C_) {
+          constexpr int c = decltype(C_)::value;
+          xr[c] = (float)A[m0 + 4 * t + c].x * su;
+          xi[c] = (float)A[m0 + 4 * t + c].y * su;
+        });
+        xpose4(xr[0], xr[1], xr[2], xr[3]);
+        xpose4(xi[0], xi[1], xi[2], xi[3]);
+        const SplitC ra = split_c(make_float2(xr[rb], xi[rb]));
+        const u32x4 Am = {ra.h, ra.l, ra.h, ra.l}, Im = {ra.hr, ra.lr, ra.hr, ra.lr};
+        static_for<rb, 4>([&](auto J_) {
+          constexpr int jb = decltype(J_)::value;
+          const SplitC cb = jb == rb ? ra : split_c(make_float2(xr[jb], xi[jb]));
+          const u32x4 Bm = {cb.h, cb.l, cb.l, cb.h};
+          dr[jb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_h8(Am), as_h8(Bm), dr[jb], 0, 0, 0);
+          di[jb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(as_h8(Im), as_h8(Bm), di[jb], 0, 0, 0);
+        });
+      });
+      static_for<0, 4>([&](auto R_) {
+        constexpr int r = decltype(R_)::value;
+        float zr[4] = {dr[0][r], dr[1][r], dr[2][r], dr[3][r]};
+        float zi[4] = {di[0][r], di[1][r], di[2][r], di[3][r]};
+        xpose4(zr[0], zr[1], zr[2], zr[3]);
+        xpose4(zi[0], zi[1], zi[2], zi[3]);
+        static_for<0, 4>([&](auto Q_) {
+          constexpr int gq = decltype(Q_)::value;
+          cplx<T>& e = A[16 * rb + 4 * gq + r];
+          e.x = fma((T)zr[gq], (T)sre, e.x);
+          e.y = fma((T)zi[gq], (T)sim, e.y);
         });
       });
     }
@@ -967,8 +1024,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         });
       };
       // BLK: the steps of the first 32 rows update rows below 32 only; the block of rows
-      // and columns 32.. takes their 32 rank-one terms at once on the matrix cores below
-      constexpr int IEND = (BLK && k < 32) ? 32 : LMAX;
+      // and columns 32.. takes their 32 rank-one terms at once on the matrix cores below.
+      // BLK2 (forward kernels on the split recursion) also leaves rows 16..31 to the first
+      // 16 steps' and rows 48..63 to steps 32..47's batched terms (sub_update)
+      constexpr int IEND = !BLK ? LMAX : (BLK2 && k < 16) ? 16 : k < 32 ? 32 : (BLK2 && k < 48) ? 48 : LMAX;
       if constexpr (SKIP) {
         // the wave steps only to its own L: rows in blocks of four with one uniform test per
         // block (a test per row cost a scalar compare and branch for every step and row);
@@ -982,6 +1041,12 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       } else {
         static_for<k + 1, IEND>(row_update);
       }
+    }
+    if constexpr (BLK2 && k == 15) {
+      if (Lw > 16) sub_update(ic<1>{});
+    }
+    if constexpr (BLK2 && k == 47) {
+      if (Lw > 48) sub_update(ic<3>{});
     }
     if constexpr (BLK && k == 31) {
       if (Lw > 32) {

@@ -34,6 +34,7 @@ namespace cosk {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Split-row length per snapshot row (elements): every read x[n + c] with n < 8 ceil(K/8),
@@ -276,6 +277,25 @@ __device__ __forceinline__ void xpose_groups(f32x16& v) {
       v[4 * h + 4 + r] = __uint_as_float(s[1]);
     }
   }
+}
+
+// A 4 x 4 transpose of 16-lane chunks over four registers: afterwards x[a] holds, in chunk
+// b, what x[b] held in chunk a (two 32-lane swaps, then two 16-lane swaps).
+__device__ __forceinline__ void xpose4(float& x0, float& x1, float& x2, float& x3) {
+  auto sw32 = [](float& x, float& y) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+  };
+  auto sw16 = [](float& x, float& y) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+  };
+  sw32(x0, x2);
+  sw32(x1, x3);
+  sw16(x0, x1);
+  sw16(x2, x3);
 }
 
 // D[i][j] -= conj(u_i) v_j for the four groups' 16 x 16 blocks (accumulator layout):
