@@ -812,7 +812,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // one pixel per wave in FP32 with 64 rows: the trailing block of rows and columns 32..63
   // is updated by the first 32 steps at once on the matrix cores
   constexpr bool BLK = std::is_same<T, float>::value && G == 64 && CPL == 1;
-  constexpr bool BLK2 = BLK && HSPLIT && !REF;
+  constexpr bool BLK2 = BLK && HSPLIT;
   constexpr int CRR = CholRegRows<G>::value;
   constexpr bool PUBLISH = !DPP_FMA && CRR < LMAX - 1;
   cplx<T>* const Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
@@ -1025,7 +1025,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       };
       // BLK: the steps of the first 32 rows update rows below 32 only; the block of rows
       // and columns 32.. takes their 32 rank-one terms at once on the matrix cores below.
-      // BLK2 (forward kernels on the split recursion) also leaves rows 16..31 to the first
+      // BLK2 (the kernels on the split recursion) also leaves rows 16..31 to the first
       // 16 steps' and rows 48..63 to steps 32..47's batched terms (sub_update)
       constexpr int IEND = !BLK ? LMAX : (BLK2 && k < 16) ? 16 : k < 32 ? 32 : (BLK2 && k < 48) ? 48 : LMAX;
       if constexpr (SKIP) {
