@@ -813,6 +813,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // is updated by the first 32 steps at once on the matrix cores
   constexpr bool BLK = std::is_same<T, float>::value && G == 64 && CPL == 1;
   constexpr bool BLK2 = BLK && HSPLIT;
+  // FP64, one pixel per wave: the same row blocks (16..31 by steps 0..15, 32..63 by 0..31,
+  // 48..63 by 32..47) batched on v_mfma_f64_16x16x4f64 (dblk_update)
+  constexpr bool DBLK = std::is_same<T, double>::value && G == 64 && CPL == 1;
   constexpr int CRR = CholRegRows<G>::value;
   constexpr bool PUBLISH = !DPP_FMA && CRR < LMAX - 1;
   cplx<T>* const Upk = reinterpret_cast<cplx<T>*>(xs_all + q * GREG);
@@ -905,6 +908,66 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           cplx<T>& e = A[16 * rb + 4 * gq + r];
           e.x = fma((T)zr[gq], (T)sre, e.x);
           e.y = fma((T)zi[gq], (T)sim, e.y);
+        });
+      });
+    }
+  };
+  // DBLK: rows 16 tr0 .. 16 tr1 - 1 (of every column from 16 tr0 on) minus the terms of
+  // steps m0 .. m0 + 2 np - 1, two steps per v_mfma_f64_16x16x4f64: A[i][k] from lane
+  // i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane j + 16 (i % 4), element i / 4.  Per
+  // step pair, X = (re, im) of rows m and m + 1 and Y = (-im, re) of the same go through a
+  // 4 x 4 transpose of 16-lane chunks (xpose4d), after which register t holds, in chunk k,
+  // the k-th of the four values of column 16t + l % 16: the A and B layout of tile t.  So
+  // Re += X_tr . X_tc and Im += Y_tr . X_tc (the conj(U[m][i]) U[m][j] terms).  For each
+  // element e of a tile row the four column tiles' results go back by the same transpose:
+  // register c then holds, in chunk tc, row 16 tr + 4e + c of column 16 tc + l % 16 (zeros
+  // for tiles left of the diagonal).
+  auto dblk_update = [&](auto TR0_, auto TR1_, auto M0_, auto NP_) {
+    if constexpr (DBLK) {
+      constexpr int tr0 = decltype(TR0_)::value, tr1 = decltype(TR1_)::value;
+      constexpr int m0 = decltype(M0_)::value, np = decltype(NP_)::value;
+      typedef double d4 __attribute__((ext_vector_type(4)));
+      d4 dre[4][4], dim[4][4];
+      static_for<tr0, tr1>([&](auto R_) {
+        constexpr int tr = decltype(R_)::value;
+        static_for<0, 4>([&](auto C_) {
+          constexpr int tc = decltype(C_)::value;
+          dre[tr][tc] = d4{0.0, 0.0, 0.0, 0.0};
+          dim[tr][tc] = d4{0.0, 0.0, 0.0, 0.0};
+        });
+      });
+      static_for<0, np>([&](auto P_) {
+        constexpr int m = m0 + 2 * decltype(P_)::value;
+        double X[4] = {(double)A[m].x, (double)A[m].y, (double)A[m + 1].x, (double)A[m + 1].y};
+        double Yv[4] = {-(double)A[m].y, (double)A[m].x, -(double)A[m + 1].y, (double)A[m + 1].x};
+        xpose4d(X[0], X[1], X[2], X[3]);
+        xpose4d(Yv[0], Yv[1], Yv[2], Yv[3]);
+        static_for<tr0, tr1>([&](auto R_) {
+          constexpr int tr = decltype(R_)::value;
+          static_for<tr, 4>([&](auto C_) {
+            constexpr int tc = decltype(C_)::value;
+            dre[tr][tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[tr], X[tc], dre[tr][tc], 0, 0, 0);
+            dim[tr][tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(Yv[tr], X[tc], dim[tr][tc], 0, 0, 0);
+          });
+        });
+      });
+      static_for<tr0, tr1>([&](auto R_) {
+        constexpr int tr = decltype(R_)::value;
+        static_for<0, 4>([&](auto E_) {
+          constexpr int e = decltype(E_)::value;
+          double qr[4], qi[4];
+          static_for<0, 4>([&](auto C_) {
+            constexpr int tc = decltype(C_)::value;
+            qr[tc] = tc >= tr ? dre[tr][tc][e] : 0.0;
+            qi[tc] = tc >= tr ? dim[tr][tc][e] : 0.0;
+          });
+          xpose4d(qr[0], qr[1], qr[2], qr[3]);
+          xpose4d(qi[0], qi[1], qi[2], qi[3]);
+          static_for<0, 4>([&](auto Q_) {
+            constexpr int c = decltype(Q_)::value;
+            A[16 * tr + 4 * e + c].x -= (T)qr[c];
+            A[16 * tr + 4 * e + c].y -= (T)qi[c];
+          });
         });
       });
     }
@@ -1027,7 +1090,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       // and columns 32.. takes their 32 rank-one terms at once on the matrix cores below.
       // BLK2 (the kernels on the split recursion) also leaves rows 16..31 to the first
       // 16 steps' and rows 48..63 to steps 32..47's batched terms (sub_update)
-      constexpr int IEND = !BLK ? LMAX : (BLK2 && k < 16) ? 16 : k < 32 ? 32 : (BLK2 && k < 48) ? 48 : LMAX;
+      constexpr int IEND = DBLK ? (k < 16 ? 16 : k < 32 ? 32 : k < 48 ? 48 : LMAX)
+                           : !BLK ? LMAX : (BLK2 && k < 16) ? 16 : k < 32 ? 32 : (BLK2 && k < 48) ? 48 : LMAX;
       if constexpr (SKIP) {
         // the wave steps only to its own L: rows in blocks of four with one uniform test per
         // block (a test per row cost a scalar compare and branch for every step and row);
@@ -1044,6 +1108,15 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     }
     if constexpr (BLK2 && k == 15) {
       if (Lw > 16) sub_update(ic<1>{});
+    }
+    if constexpr (DBLK && k == 15) {
+      if (Lw > 16) dblk_update(ic<1>{}, ic<2>{}, ic<0>{}, ic<8>{});
+    }
+    if constexpr (DBLK && k == 31) {
+      if (Lw > 32) dblk_update(ic<2>{}, ic<4>{}, ic<0>{}, ic<16>{});
+    }
+    if constexpr (DBLK && k == 47) {
+      if (Lw > 48) dblk_update(ic<3>{}, ic<4>{}, ic<32>{}, ic<8>{});
     }
     if constexpr (BLK2 && k == 47) {
       if (Lw > 48) sub_update(ic<3>{});

@@ -298,6 +298,24 @@ __device__ __forceinline__ void xpose4(float& x0, float& x1, float& x2, float& x
   sw16(x2, x3);
 }
 
+// The same transpose for doubles (each half of the 64-bit value moved alike).
+__device__ __forceinline__ void xpose4d(double& x0, double& x1, double& x2, double& x3) {
+  auto lo = [](double v) { return __uint_as_float((unsigned)__builtin_bit_cast(unsigned long long, v)); };
+  auto hi = [](double v) { return __uint_as_float((unsigned)(__builtin_bit_cast(unsigned long long, v) >> 32)); };
+  auto mk = [](float l, float h) {
+    return __builtin_bit_cast(double, (unsigned long long)__float_as_uint(l) |
+                                          ((unsigned long long)__float_as_uint(h) << 32));
+  };
+  float l0 = lo(x0), l1 = lo(x1), l2 = lo(x2), l3 = lo(x3);
+  float h0 = hi(x0), h1 = hi(x1), h2 = hi(x2), h3 = hi(x3);
+  xpose4(l0, l1, l2, l3);
+  xpose4(h0, h1, h2, h3);
+  x0 = mk(l0, h0);
+  x1 = mk(l1, h1);
+  x2 = mk(l2, h2);
+  x3 = mk(l3, h3);
+}
+
 // D[i][j] -= conj(u_i) v_j for the four groups' 16 x 16 blocks (accumulator layout):
 // Re -= ur_i vr_j + ui_i vi_j, Im -= ur_i vi_j - ui_i vr_j, u from the block's rows, v from
 // its columns (row k of the respective slots); one of the four MFMAs per call, J = 0..3.
