@@ -869,15 +869,17 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // below 2^14 (|U[m][c]|^2 <= tr S_L) and the sums applied times 2^(-2 eu).  The results
   // go back by the same transpose: register g of set r then holds, in chunk jb, row
   // 16 rb + 4g + r of column 16 jb + l % 16 (chunks below rb receive zeros).
-  auto sub_update = [&](auto RB_) {
+  auto sub_update = [&](auto RB_, auto R0_, auto M0_, auto NT_) {
     if constexpr (BLK2) {
-      constexpr int rb = decltype(RB_)::value, m0 = 16 * (rb - 1);
+      // rows 16 rb + r0 .. 16 rb + 15 by steps m0 .. m0 + 4 nt - 1
+      constexpr int rb = decltype(RB_)::value, r0 = decltype(R0_)::value;
+      constexpr int m0 = decltype(M0_)::value, nt = decltype(NT_)::value;
       int et = 0;
       (void)frexpf((float)(trS + (T)L * delta), &et);   // tr S_L < 2^et
       const int eu = (trS > T(0) && trS < T(3.0e38)) ? 14 - ((et + 1) >> 1) : 0;
       const float su = ldexpf(1.f, eu), sre = -ldexpf(1.f, -2 * eu), sim = -sre;
       f32x4v dr[4] = {}, di[4] = {};
-      static_for<0, 4>([&](auto T_) {
+      static_for<0, nt>([&](auto T_) {
         constexpr int t = decltype(T_)::value;
         float xr[4], xi[4];
         static_for<0, 4>([&](auto C_) {
@@ -905,9 +907,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         xpose4(zi[0], zi[1], zi[2], zi[3]);
         static_for<0, 4>([&](auto Q_) {
           constexpr int gq = decltype(Q_)::value;
-          cplx<T>& e = A[16 * rb + 4 * gq + r];
-          e.x = fma((T)zr[gq], (T)sre, e.x);
-          e.y = fma((T)zi[gq], (T)sim, e.y);
+          if constexpr (4 * gq + r >= r0) {   // the tile's rows above r0 are not this update's
+            cplx<T>& e = A[16 * rb + 4 * gq + r];
+            e.x = fma((T)zr[gq], (T)sre, e.x);
+            e.y = fma((T)zi[gq], (T)sim, e.y);
+          }
         });
       });
     }
@@ -1091,7 +1095,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       // BLK2 (the kernels on the split recursion) also leaves rows 16..31 to the first
       // 16 steps' and rows 48..63 to steps 32..47's batched terms (sub_update)
       constexpr int IEND = DBLK ? (k < 16 ? 16 : k < 32 ? 32 : k < 48 ? 48 : LMAX)
-                           : !BLK ? LMAX : (BLK2 && k < 16) ? 16 : k < 32 ? 32 : (BLK2 && k < 48) ? 48 : LMAX;
+                           : BLK2 ? 8 * (k / 8 + 1)
+                                  : !BLK ? LMAX : k < 32 ? 32 : LMAX;
       if constexpr (SKIP) {
         // the wave steps only to its own L: rows in blocks of four with one uniform test per
         // block (a test per row cost a scalar compare and branch for every step and row);
@@ -1106,8 +1111,13 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
         static_for<k + 1, IEND>(row_update);
       }
     }
+    // BLK2: the second half of each 16-row block by the first half's 8 steps, then the
+    // second 16-row block of each half by the first's 16 steps (BLK takes the rest)
+    if constexpr (BLK2 && k % 16 == 7) {
+      if (Lw > k + 1) sub_update(ic<k / 16>{}, ic<8>{}, ic<k - 7>{}, ic<2>{});
+    }
     if constexpr (BLK2 && k == 15) {
-      if (Lw > 16) sub_update(ic<1>{});
+      if (Lw > 16) sub_update(ic<1>{}, ic<0>{}, ic<0>{}, ic<4>{});
     }
     if constexpr (DBLK && k == 15) {
       if (Lw > 16) dblk_update(ic<1>{}, ic<2>{}, ic<0>{}, ic<8>{});
@@ -1119,7 +1129,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       if (Lw > 48) dblk_update(ic<3>{}, ic<4>{}, ic<32>{}, ic<8>{});
     }
     if constexpr (BLK2 && k == 47) {
-      if (Lw > 48) sub_update(ic<3>{});
+      if (Lw > 48) sub_update(ic<3>{}, ic<0>{}, ic<32>{}, ic<4>{});
     }
     if constexpr (BLK && k == 31) {
       if (Lw > 32) {
