@@ -813,8 +813,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // is updated by the first 32 steps at once on the matrix cores
   constexpr bool BLK = std::is_same<T, float>::value && G == 64 && CPL == 1;
   constexpr bool BLK2 = BLK && HSPLIT;
-  // FP64, one pixel per wave: the same row blocks (16..31 by steps 0..15, 32..63 by 0..31,
-  // 48..63 by 32..47) batched on v_mfma_f64_16x16x4f64 (dblk_update)
+  // FP64, one pixel per wave: the same row blocks (the second half of each 16-row block by
+  // the first half's steps, 16..31 by steps 0..15, 32..63 by 0..31, 48..63 by 32..47)
+  // batched on v_mfma_f64_16x16x4f64 (dblk_update)
   constexpr bool DBLK = std::is_same<T, double>::value && G == 64 && CPL == 1;
   constexpr int CRR = CholRegRows<G>::value;
   constexpr bool PUBLISH = !DPP_FMA && CRR < LMAX - 1;
@@ -926,10 +927,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // element e of a tile row the four column tiles' results go back by the same transpose:
   // register c then holds, in chunk tc, row 16 tr + 4e + c of column 16 tc + l % 16 (zeros
   // for tiles left of the diagonal).
-  auto dblk_update = [&](auto TR0_, auto TR1_, auto M0_, auto NP_) {
+  auto dblk_update = [&](auto TR0_, auto TR1_, auto M0_, auto NP_, auto R0_) {
     if constexpr (DBLK) {
+      // (rows of tile tr0 below 16 tr0 + r0 are not this update's)
       constexpr int tr0 = decltype(TR0_)::value, tr1 = decltype(TR1_)::value;
-      constexpr int m0 = decltype(M0_)::value, np = decltype(NP_)::value;
+      constexpr int m0 = decltype(M0_)::value, np = decltype(NP_)::value, r0 = decltype(R0_)::value;
       typedef double d4 __attribute__((ext_vector_type(4)));
       d4 dre[4][4], dim[4][4];
       static_for<tr0, tr1>([&](auto R_) {
@@ -969,8 +971,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           xpose4d(qi[0], qi[1], qi[2], qi[3]);
           static_for<0, 4>([&](auto Q_) {
             constexpr int c = decltype(Q_)::value;
-            A[16 * tr + 4 * e + c].x -= (T)qr[c];
-            A[16 * tr + 4 * e + c].y -= (T)qi[c];
+            if constexpr (tr > tr0 || 4 * e + c >= r0) {
+              A[16 * tr + 4 * e + c].x -= (T)qr[c];
+              A[16 * tr + 4 * e + c].y -= (T)qi[c];
+            }
           });
         });
       });
@@ -1094,7 +1098,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       // and columns 32.. takes their 32 rank-one terms at once on the matrix cores below.
       // BLK2 (the kernels on the split recursion) also leaves rows 16..31 to the first
       // 16 steps' and rows 48..63 to steps 32..47's batched terms (sub_update)
-      constexpr int IEND = DBLK ? (k < 16 ? 16 : k < 32 ? 32 : k < 48 ? 48 : LMAX)
+      constexpr int IEND = DBLK ? 8 * (k / 8 + 1)
                            : BLK2 ? 8 * (k / 8 + 1)
                                   : !BLK ? LMAX : k < 32 ? 32 : LMAX;
       if constexpr (SKIP) {
@@ -1119,14 +1123,17 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     if constexpr (BLK2 && k == 15) {
       if (Lw > 16) sub_update(ic<1>{}, ic<0>{}, ic<0>{}, ic<4>{});
     }
+    if constexpr (DBLK && k % 16 == 7) {
+      if (Lw > k + 1) dblk_update(ic<k / 16>{}, ic<k / 16 + 1>{}, ic<k - 7>{}, ic<4>{}, ic<8>{});
+    }
     if constexpr (DBLK && k == 15) {
-      if (Lw > 16) dblk_update(ic<1>{}, ic<2>{}, ic<0>{}, ic<8>{});
+      if (Lw > 16) dblk_update(ic<1>{}, ic<2>{}, ic<0>{}, ic<8>{}, ic<0>{});
     }
     if constexpr (DBLK && k == 31) {
-      if (Lw > 32) dblk_update(ic<2>{}, ic<4>{}, ic<0>{}, ic<16>{});
+      if (Lw > 32) dblk_update(ic<2>{}, ic<4>{}, ic<0>{}, ic<16>{}, ic<0>{});
     }
     if constexpr (DBLK && k == 47) {
-      if (Lw > 48) dblk_update(ic<3>{}, ic<4>{}, ic<32>{}, ic<8>{});
+      if (Lw > 48) dblk_update(ic<3>{}, ic<4>{}, ic<32>{}, ic<8>{}, ic<0>{});
     }
     if constexpr (BLK2 && k == 47) {
       if (Lw > 48) sub_update(ic<3>{}, ic<0>{}, ic<32>{}, ic<4>{});
