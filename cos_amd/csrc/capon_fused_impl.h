@@ -857,10 +857,11 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
       });
     }
   };
-  // BLK2: rows 16 rb .. 16 rb + 15 of every column 16 rb.. minus the 16 rank-one terms of
-  // steps 16 (rb - 1) .. 16 rb - 1, on v_mfma_f32_16x16x32_f16 (one 16 x 16 column tile jb
-  // per instruction; inner index k = 8 (l / 16) + s, so each 16-lane chunk g of the wave
-  // carries one step).  For four steps 4t + g at a time, a 4 x 4 transpose of 16-lane
+  // BLK2: rows 16 rb + r0 .. 16 rb + 15 of every column 16 rb.. minus the rank-one terms of
+  // steps m0 .. m0 + 4 nt - 1 (the second 16-row block of a half by the first's 16 steps,
+  // or the second half of a 16-row block by the first half's 8), on
+  // v_mfma_f32_16x16x32_f16 (one 16 x 16 column tile jb per instruction; inner index
+  // k = 8 (l / 16) + s, so each 16-lane chunk g of the wave carries one step).  For four steps 4t + g at a time, a 4 x 4 transpose of 16-lane
   // chunks (xpose4) of the step rows leaves, in register jb, column tile jb's values of
   // step 4t + g in chunk g: the layout of both operands, rows (tile rb) and columns (tile
   // jb).  Each (re, im) pair is split hi + lo in FP16 with
@@ -869,10 +870,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // (im, -re), which gives -Im.  The operands are scaled by 2^eu with 2^eu sqrt(tr S_L)
   // below 2^14 (|U[m][c]|^2 <= tr S_L) and the sums applied times 2^(-2 eu).  The results
   // go back by the same transpose: register g of set r then holds, in chunk jb, row
-  // 16 rb + 4g + r of column 16 jb + l % 16 (chunks below rb receive zeros).
+  // 16 rb + 4g + r of column 16 jb + l % 16 (chunks below rb receive zeros; rows of the
+  // tile above r0 are left as they are).
   auto sub_update = [&](auto RB_, auto R0_, auto M0_, auto NT_) {
     if constexpr (BLK2) {
-      // rows 16 rb + r0 .. 16 rb + 15 by steps m0 .. m0 + 4 nt - 1
       constexpr int rb = decltype(RB_)::value, r0 = decltype(R0_)::value;
       constexpr int m0 = decltype(M0_)::value, nt = decltype(NT_)::value;
       int et = 0;
