@@ -1346,30 +1346,33 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           constexpr int S = decltype(S_)::value;
           sy[S] = {T(0), T(0)};
         });
-        for (int t = 0; t < TW; ++t) {
-          const float2* xt = srcw + (size_t)t * a.Mstride;
-          for (int kb = 0; kb < K; kb += G) {
-            if (K - kb < G / 4) {
-              // a short last block (K = 33 at L = 32 leaves one subarray): each beta_k
-              // as a group sum over the lanes' columns, x_t[k + c] loaded once for both
-              for (int k = kb; k < K; ++k) {
-                cplx<T> xk[CPL];
-                T br = T(0), bi = T(0);
-                static_for<0, CPL>([&](auto S_) {
-                  constexpr int S = decltype(S_)::value;
-                  xk[S] = ldx<T>(xt + min(k + cs[S], M - 1));
-                  br = fma(xk[S].x, y[S].x, fma(xk[S].y, y[S].y, br));
-                  bi = fma(xk[S].x, y[S].y, fma(-xk[S].y, y[S].x, bi));
-                });
-                const cplx<T> bk = {Gp::sum(br), Gp::sum(bi)};
-                static_for<0, CPL>([&](auto S_) {
-                  constexpr int S = decltype(S_)::value;
-                  add_mul(sy[S], xk[S], bk);
-                });
+        // a short last block (K = 33 at L = 32 leaves one subarray): each beta_k as a group
+        // sum over the lanes' columns, x_t[k + c] loaded once for both
+        auto short_block = [&](const float2* xt, int kb) {
+          for (int k = kb; k < K; ++k) {
+            cplx<T> xk[CPL];
+            T br = T(0), bi = T(0);
+            static_for<0, CPL>([&](auto S_) {
+              constexpr int S = decltype(S_)::value;
+              xk[S] = ldx<T>(xt + min(k + cs[S], M - 1));
+              br = fma(xk[S].x, y[S].x, fma(xk[S].y, y[S].y, br));
+              bi = fma(xk[S].x, y[S].y, fma(-xk[S].y, y[S].x, bi));
+            });
+            const cplx<T> bk = {Gp::sum(br), Gp::sum(bi)};
+            static_for<0, CPL>([&](auto S_) {
+              constexpr int S = decltype(S_)::value;
+              add_mul(sy[S], xk[S], bk);
+            });
+          }
+        };
+        if constexpr (G == 64) {
+          for (int t = 0; t < TW; ++t) {
+            const float2* xt = srcw + (size_t)t * a.Mstride;
+            for (int kb = 0; kb < K; kb += G) {
+              if (K - kb < G / 4) {
+                short_block(xt, kb);
+                break;
               }
-              break;
-            }
-            if constexpr (G == 64) {
               // one pixel per wave: y_l from LDS (one broadcast read each, nothing held in
               // scalar registers) and the window in two halves of 32, the low half loaded
               // again for the column terms, so the refinement stays within the
@@ -1418,31 +1421,90 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
                 load_half(0);
               }
               cols_half(ic<0>{});
-            } else {
-              cplx<T> xv[LMAX];
-              static_for<0, LMAX>([&](auto l_) {
-                constexpr int l = decltype(l_)::value;
-                xv[l] = ldx<T>(xt + min(kb + g + l, M - 1));
-              });
-              cplx<T> nb = {T(0), T(0)};   // -(y^H u_{t,k}) = -conj(beta_{t,k})
-              // the broadcast sources (y, then nb) are written before each loop: only their
-              // first reads need the DPP wait states
-              static_for<0, LMAX>([&](auto l_) {
-                constexpr int l = decltype(l_)::value;
-                bcast_sub_conj_mul<G, l % G, l % G == 0>(nb, y[l / G], xv[l]);
-              });
-              if (kb + g >= K) nb = {T(0), T(0)};
-              // (S y)_c += x_t[k + c] beta_{t,k} for the G subarrays of the block
-              static_for<0, G>([&](auto l2_) {
-                constexpr int l2 = decltype(l2_)::value;
-                if constexpr (CPL >= 2)
-                  bcast_sub_conj_mul2<G, l2, l2 == 0>(sy[0], sy[1], nb, xv[l2], xv[l2 + G]);
-                else
-                  bcast_sub_conj_mul<G, l2, l2 == 0>(sy[0], nb, xv[l2]);
-                if constexpr (CPL == 3)   // after the pair's FMAs, which do not write nb
-                  bcast_sub_conj_mul<G, l2, false>(sy[2], nb, xv[l2 + 2 * G]);
-              });
             }
+          }
+        } else {
+          // The full blocks of G subarrays (kb <= K - G/4) whose windows lie inside the row
+          // (kb + G - 1 + LMAX <= M, so the loads take constant offsets) form one sequence
+          // over all window rows, software-pipelined: the next block's window is loaded into
+          // the other register buffer before the current block is computed, so its load
+          // latency overlaps the FMAs (after the factorisation the triangle's registers are
+          // free; one column per lane keeps a single buffer).  Past the last block the loads
+          // repeat it (cache-resident, unused).  The blocks that reach the row's end follow
+          // with clamped loads, then the short ones.
+          const int nf = K >= G / 4 ? (K - G / 4) / G + 1 : 0;   // full blocks per row
+          const int nin = min(nf, M - LMAX - G + 1 >= 0 ? (M - LMAX - G + 1) / G + 1 : 0);
+          const int nblk = TW * nin;
+          cplx<T> xa[LMAX], xb[LMAX];
+          int lt = 0, lkb = 0;   // the next block to load
+          auto load_in = [&](cplx<T> (&xv)[LMAX]) {
+            const float2* const xp = srcw + (size_t)lt * a.Mstride + lkb + g;
+            static_for<0, LMAX>([&](auto l_) {
+              constexpr int l = decltype(l_)::value;
+              xv[l] = ldx<T>(xp + l);
+            });
+            if (lt * nin + lkb / G + 1 < nblk) {
+              lkb += G;
+              if (lkb == nin * G) {
+                lkb = 0;
+                ++lt;
+              }
+            }
+          };
+          auto comp_blk = [&](const cplx<T> (&xv)[LMAX], int kb) {
+            cplx<T> nb = {T(0), T(0)};   // -(y^H u_{t,k}) = -conj(beta_{t,k})
+            // the broadcast sources (y, then nb) are written before each loop: only their
+            // first reads need the DPP wait states
+            static_for<0, LMAX>([&](auto l_) {
+              constexpr int l = decltype(l_)::value;
+              bcast_sub_conj_mul<G, l % G, l % G == 0>(nb, y[l / G], xv[l]);
+            });
+            if (kb + g >= K) nb = {T(0), T(0)};
+            // (S y)_c += x_t[k + c] beta_{t,k} for the G subarrays of the block
+            static_for<0, G>([&](auto l2_) {
+              constexpr int l2 = decltype(l2_)::value;
+              if constexpr (CPL >= 2)
+                bcast_sub_conj_mul2<G, l2, l2 == 0>(sy[0], sy[1], nb, xv[l2], xv[l2 + G]);
+              else
+                bcast_sub_conj_mul<G, l2, l2 == 0>(sy[0], nb, xv[l2]);
+              if constexpr (CPL == 3)   // after the pair's FMAs, which do not write nb
+                bcast_sub_conj_mul<G, l2, false>(sy[2], nb, xv[l2 + 2 * G]);
+            });
+          };
+          int ckb = 0;   // the block computed next
+          auto next_ckb = [&]() {
+            ckb += G;
+            if (ckb == nin * G) ckb = 0;
+          };
+          if constexpr (CPL == 1) {   // one column per lane: 128 registers, one buffer
+            for (int i = 0; i < nblk; ++i) {
+              load_in(xa);
+              comp_blk(xa, ckb);
+              next_ckb();
+            }
+          } else {
+            if (nblk > 0) load_in(xa);
+            for (int i = 0; i < nblk; i += 2) {
+              load_in(xb);
+              comp_blk(xa, ckb);
+              next_ckb();
+              load_in(xa);
+              if (i + 1 < nblk) {
+                comp_blk(xb, ckb);
+                next_ckb();
+              }
+            }
+          }
+          for (int t = 0; t < TW; ++t) {
+            const float2* const xt = srcw + (size_t)t * a.Mstride;
+            for (int kb = nin * G; kb < nf * G; kb += G) {
+              static_for<0, LMAX>([&](auto l_) {
+                constexpr int l = decltype(l_)::value;
+                xa[l] = ldx<T>(xt + min(kb + g + l, M - 1));
+              });
+              comp_blk(xa, kb);
+            }
+            if (nf * G < K) short_block(xt, nf * G);
           }
         }
         // the right-hand side a: all ones, or the steering vector exp(j pi c sin(theta))
@@ -1467,6 +1529,10 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           r[S] = cs[S] < L ? cplx<T>{av[S].x - sy[S].x - delta * y[S].x, av[S].y - sy[S].y - delta * y[S].y}
                            : cplx<T>{T(0), T(0)};
         });
+        // the columns' LDS offsets through an opaque copy: the solves below recompute their
+        // Upk addresses instead of holding the first solves' ones across the residual loop
+#pragma unroll
+        for (int S = 0; S < CPL; ++S) asm volatile("" : "+v"(cs[S]), "+v"(ucol[S]));
         fwd_upk(r);
         cplx<T> cr2[CPL], dy[CPL];
         static_for<0, CPL>([&](auto S_) {
