@@ -1229,12 +1229,13 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // Forward solve U^H v = b with the rows of U read back from Upk; v_k goes to the
   // diagonal slot of column k.  Lanes whose column is <= k read the dump slot and change
   // only right-hand sides already consumed.
+  int io = 0;   // 0; made opaque where the solves must not reuse earlier reads of invd
   auto fwd_upk = [&](cplx<T>(&bs)[CPL]) {
     static_for<0, LMAX>([&](auto Kk_) {
       constexpr int k = decltype(Kk_)::value, sk = k / G, lk = k % G;
       if (SKIP && k >= Lw) return;
       cplx<T> v1 = bcastc<G, lk>(bs[sk]);
-      const T inv = invd[k];
+      const T inv = invd[k + io];
       v1.x *= inv;
       v1.y *= inv;
       static_for<0, CPL>([&](auto S_) {
@@ -1258,7 +1259,7 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     static_for<LMAX - 1, 0, -1>([&](auto Cc_) {
       constexpr int jj = decltype(Cc_)::value, sj = jj / G, lj = jj % G;
       if (SKIP && jj >= Lw) return;
-      const T inv = invd[jj];
+      const T inv = invd[jj + io];
       cplx<T> yb = bcastc<G, lj>(cr[sj]);
       yb.x *= inv;
       yb.y *= inv;
@@ -1374,53 +1375,60 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
                 break;
               }
               // one pixel per wave: y_l from LDS (one broadcast read each, nothing held in
-              // scalar registers) and the window in two halves of 32, the low half loaded
-              // again for the column terms, so the refinement stays within the
-              // factorisation's register budget.  beta needs l < L, the column terms
-              // l2 < K - kb (both uniform).
-              const int nk = __builtin_amdgcn_readfirstlane(K - kb);
+              // scalar registers) and the window in quarters of 16 through two register
+              // buffers, each quarter's loads issued one quarter ahead of its products:
+              // beta over quarters 0..3, then the column terms over 3, 2 (still held), 0, 1
+              // (loaded again).  Every lane takes all 64 terms with no per-term test: y_l
+              // is 0 for l >= L and beta_k for k >= K (written so below), and the clamped
+              // loads stay in the row, so the extra terms add exact zeros.
               // an offset the compiler cannot see through keeps the y reads inside the
               // loop (hoisted, the 64 values would hold 128 VGPRs across it)
               int yo = 0;
               asm volatile("" : "+v"(yo));
               const cplx<T>* const yl = rowb + yo;
-              cplx<T> xv[32];
+              const float2* const xp = xt + kb + g;
+              const int lim = M - 1 - kb - g;   // the row's last element, from xp
+              cplx<T> xa[16], xb[16];
               cplx<T> nb = {T(0), T(0)};
-              auto load_half = [&](int h) {
-                static_for<0, 32>([&](auto l_) {
+              auto load_q = [&](cplx<T> (&xv)[16], auto H_) {
+                constexpr int h = decltype(H_)::value;
+                static_for<0, 16>([&](auto l_) {
                   constexpr int l = decltype(l_)::value;
-                  if (h + l < Lw || h + l < nk) xv[l] = ldx<T>(xt + min(kb + g + h + l, M - 1));
+                  xv[l] = ldx<T>(xp + min(h + l, lim));
                 });
               };
-              auto beta_half = [&](int h) {
-                static_for<0, 32>([&](auto l_) {
+              auto beta_q = [&](const cplx<T> (&xv)[16], auto H_) {
+                constexpr int h = decltype(H_)::value;
+                static_for<0, 16>([&](auto l_) {
                   constexpr int l = decltype(l_)::value;
-                  if (h + l < Lw) sub_conj_mul(nb, yl[h + l], xv[l]);
+                  sub_conj_mul(nb, yl[h + l], xv[l]);
                 });
               };
               // beta_k reaches the column owners through LDS as well (broadcast reads)
               const cplx<T>* const bl = rowb + LMAX + yo;
-              auto cols_half = [&](auto H_) {
+              auto cols_q = [&](const cplx<T> (&xv)[16], auto H_) {
                 constexpr int h = decltype(H_)::value;
-                static_for<0, 32>([&](auto l_) {
+                static_for<0, 16>([&](auto l_) {
                   constexpr int l2 = decltype(l_)::value;
-                  if (h + l2 < nk) sub_conj_mul(sy[0], bl[h + l2], xv[l2]);
+                  sub_conj_mul(sy[0], bl[h + l2], xv[l2]);
                 });
               };
-              load_half(0);
-              beta_half(0);
-              const bool two = Lw > 32 || nk > 32;
-              if (two) {
-                load_half(32);
-                beta_half(32);
-              }
+              load_q(xa, ic<0>{});
+              load_q(xb, ic<16>{});
+              beta_q(xa, ic<0>{});
+              load_q(xa, ic<32>{});
+              beta_q(xb, ic<16>{});
+              load_q(xb, ic<48>{});
+              beta_q(xa, ic<32>{});
+              beta_q(xb, ic<48>{});
               if (kb + g >= K) nb = {T(0), T(0)};
               rowb[LMAX + g] = nb;
-              if (two) {
-                cols_half(ic<32>{});
-                load_half(0);
-              }
-              cols_half(ic<0>{});
+              cols_q(xb, ic<48>{});
+              load_q(xb, ic<0>{});
+              cols_q(xa, ic<32>{});
+              load_q(xa, ic<16>{});
+              cols_q(xb, ic<0>{});
+              cols_q(xa, ic<16>{});
             }
           }
         } else {
@@ -1529,10 +1537,12 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
           r[S] = cs[S] < L ? cplx<T>{av[S].x - sy[S].x - delta * y[S].x, av[S].y - sy[S].y - delta * y[S].y}
                            : cplx<T>{T(0), T(0)};
         });
-        // the columns' LDS offsets through an opaque copy: the solves below recompute their
-        // Upk addresses instead of holding the first solves' ones across the residual loop
+        // the columns' LDS offsets and the 1/U_kk index through opaque copies: the solves
+        // below recompute their addresses and read 1/U_kk again instead of holding the first
+        // solves' addresses and values across the residual loop
 #pragma unroll
         for (int S = 0; S < CPL; ++S) asm volatile("" : "+v"(cs[S]), "+v"(ucol[S]));
+        asm volatile("" : "+v"(io));
         fwd_upk(r);
         cplx<T> cr2[CPL], dy[CPL];
         static_for<0, CPL>([&](auto S_) {
@@ -1771,10 +1781,11 @@ __device__ __noinline__ void capon_group_call(const CaponLaunch& a, const int MP
   capon_group<T, G, CPL, TM, DIRECT, BWD, REF>(a, MP, GREG, gid0, npix);
 }
 
-// Waves per block (see capon_group): four for the FP32 64-lane forward kernels with a fixed
-// window bound, one otherwise.
+// Waves per block (see capon_group): four for the FP32 64-lane forward and refinement
+// kernels with a fixed window bound (the block's waves share their window rows), one
+// otherwise.
 template <typename T, int G, int TM, bool PERSIST, int BWD, bool REF> struct Wpb {
-  static constexpr int value = (std::is_same<T, float>::value && G == 64 && TM > 0 && !PERSIST && !BWD && !REF) ? 4 : 1;
+  static constexpr int value = (std::is_same<T, float>::value && G == 64 && TM > 0 && !PERSIST && !BWD) ? 4 : 1;
 };
 
 template <typename T, int G, int CPL, int TM, bool DIRECT, bool PERSIST, int BWD = 0, bool REF = false,

@@ -302,12 +302,14 @@ def test_refinement_on_ill_conditioned_covariance():
     assert np.median(we(o1["w"])) <= np.median(we(o0["w"]))
 
 
-@pytest.mark.parametrize("M,L,Y", [(24, 12, 1), (48, 24, 2), (80, 40, 1), (128, 64, 2), (16, 1, 2), (32, 32, 1),
-                                   (20, 20, 0)])
+@pytest.mark.parametrize("M,L,Y", [(24, 12, 1), (48, 24, 2), (80, 40, 1), (128, 64, 2), (128, 64, 1), (16, 1, 2),
+                                   (32, 32, 1), (20, 20, 0)])
 def test_refined_shapes_match_oracle(M, L, Y):
     """The refined kernel in the default lane shape of each L (16 lanes x 1, 2 or 3 columns,
     64 lanes) against the FP64 oracle: P within 1e-4, w within 1e-3 of its largest entry,
-    z within 1e-4."""
+    z within 1e-4.  At L = 64 (blocks of four 64-lane waves) a beam of 14 rows holds 10
+    pixels with Y = 2, so some blocks straddle two beams and stage their rows per wave, and
+    12 with Y = 1, so every block shares its window rows."""
     say("refine shape", M, L, Y)
     x = uniform_cube(3 * M + L, 3, 14, M)
     o, _ = _plan_run(x, 0.05, L, Y, refine=True)
