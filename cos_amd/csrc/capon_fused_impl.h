@@ -674,8 +674,9 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
   // ---------------- trace, loading, padding ----------------
   // tr(S) = sum_t sum_m |x_t[m]|^2 * #{(k, l): k + l = m, k < K, l < L}
   T trS;
-  if (mg) {
-    // the diagonal of the Gram sum (L is the same in the wave's four groups)
+  if (mg || mr) {
+    // the diagonal of the Gram sum or of the matrix-core recursion's S (L is the same in
+    // the wave's four groups; both were built from the scaled rows, as is the trace)
     T tr = T(0);
     static_for<0, LMAX>([&](auto K_) {
       constexpr int k = decltype(K_)::value;
