@@ -680,7 +680,8 @@ __device__ __forceinline__ void capon_group(const CaponLaunch& a, const int MP, 
     T tr = T(0);
     static_for<0, LMAX>([&](auto K_) {
       constexpr int k = decltype(K_)::value;
-      if (k < L) tr += Gp::template bcast<k % G>(A[slot<G, k / G, k>()].x);
+      const T dk = Gp::template bcast<k % G>(A[slot<G, k / G, k>()].x);
+      tr += k < L ? dk : T(0);   // a select: a branch per column would split the wave
     });
     trS = tr;
   } else {
