@@ -178,7 +178,9 @@ __device__ __forceinline__ int stage_rows_scaled(float2* dst, int MP, const floa
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
       const int m = l + 64 * c;
-      v[r][c] = (r < nrows && m < M) ? src[(size_t)r * sstride + m] : make_float2(0.f, 0.f);
+      // every lane loads (a clamped, valid address) and selects: no branch per lane
+      const float2 t = src[(size_t)min(r, nrows - 1) * sstride + min(m, M - 1)];
+      v[r][c] = (r < nrows && m < M) ? t : make_float2(0.f, 0.f);
       mx = fmaxf(mx, fmaxf(fabsf(v[r][c].x), fabsf(v[r][c].y)));
     }
   mx = wave_max(mx);
@@ -191,7 +193,8 @@ __device__ __forceinline__ int stage_rows_scaled(float2* dst, int MP, const floa
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
       const int m = l + 64 * c;
-      if (r < nrows && m < MP) dst[r * MP + m] = make_float2(v[r][c].x * sc, v[r][c].y * sc);
+      // lanes past the padded row store their zero into its last padding slot (MP > M)
+      if (r < nrows) dst[r * MP + min(m, MP - 1)] = make_float2(v[r][c].x * sc, v[r][c].y * sc);
     }
   if constexpr (ZTAIL)
     for (int m = 64 * CB + l; m < MP; m += 64)
